@@ -1,0 +1,128 @@
+/*
+ * C ABI of the MI355X (gfx950) CPA-integration library, libcpab_hip.so.
+ *
+ * Every pointer argument except cpab_tess_t itself and the small host arrays
+ * dsize / outsize / nc is a device pointer.  The library never allocates:
+ * outputs, gradients and the workspace (size from cpab_workspace_bytes) are
+ * owned by the caller, and nothing is zero-filled inside the library except
+ * where stated (grad_data must arrive zero-filled).  Every launch goes to the
+ * given stream; no call synchronises the host.  Return value: CPAB_OK (0) or a
+ * CPAB_ERR_* code, text from cpab_status_string().
+ *
+ * Layouts follow the reference's pytorch backend:
+ *   theta  [n_theta, d] f32; B [D, d] f32 with D = nC * ndim * (ndim+1)
+ *   points [ndim, nP] (shared grid) or [n_theta, ndim, nP] (per_theta_grid=1)
+ *   out    [n_theta, ndim, nP]
+ *   data   [N, C, s0(, s1(, s2))], interpolated output [N, C, out0(, out1(, out2))]
+ *   grid point p of an outsize grid is x-fastest: p = i0 + out0*(i1 + out1*i2)
+ */
+#ifndef CPAB_HIP_H
+#define CPAB_HIP_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPAB_ABI_VERSION 1
+
+#define CPAB_OK 0
+#define CPAB_ERR_ARG 1
+#define CPAB_ERR_NDIM 2
+#define CPAB_ERR_WORKSPACE 3
+#define CPAB_ERR_HIP 4
+#define CPAB_ERR_LDS 5
+#define CPAB_ERR_UNSUPPORTED 6
+
+/* Tessellation on the device.  planes [nC, 4, 4] f32 (n0, n1, n2, e) unit
+ * outward facet planes in q = x*nc space, nbr [nC, 4] i32 cell across each
+ * facet (<0: none).  Both may be NULL for ndim == 1.  Replaces the
+ * (params.nc, params.basis) pair that libcpab/pytorch/transformer.py:146-159
+ * ships to the device on every call. */
+typedef struct cpab_tess {
+    int ndim;
+    int nc[3];
+    int nC;
+    int d;
+    const float* B;
+    const float* planes;
+    const int* nbr;
+} cpab_tess_t;
+
+const char* cpab_status_string(int status);
+int cpab_abi_version(void);
+
+/* Workspace for one transform/transform_data call on n_theta x nP points. */
+size_t cpab_workspace_bytes(const cpab_tess_t* t, int n_theta, int nP);
+
+/* Floats in the forward's saved state for the fused backward: n_theta*(ndim+ndim^2)*nP. */
+size_t cpab_saved_floats(int ndim, int n_theta, int nP);
+
+/* A_q = B theta per cell, in q-space.  Replaces the basis expansion of
+ * libcpab/pytorch/transformer.py:146-150. */
+int cpab_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, float* Aq, hipStream_t stream);
+
+/* Closed-form integration of the points for time t_end (1 in the Cpab API).
+ * jac (nullable) receives the q-space Jacobian [n_theta, ndim*ndim, nP] used by
+ * cpab_transform_bwd to skip its first pass.  Replaces cpab_gpu.forward,
+ * libcpab/pytorch/transformer_cuda.cpp:17-41 (+ the Trels expm of
+ * libcpab/pytorch/transformer.py:152-155). */
+int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
+                       int per_theta_grid, float t_end, float* out, float* jac, void* workspace, size_t ws_bytes,
+                       hipStream_t stream);
+
+/* grad_theta [n_theta, d] = d<grad_out, transform(points)>/d theta, and (if
+ * grad_points != NULL) its gradient w.r.t. the points [n_theta, ndim, nP].
+ * Replaces cpab_gpu.backward (libcpab/pytorch/transformer_cuda.cpp:43-70) plus
+ * the host-side reduction of libcpab/pytorch/transformer.py:201; the
+ * reference returns no point gradient (transformer.py:202). */
+int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
+                       int per_theta_grid, float t_end, const float* grad_out, const float* jac, float* grad_theta,
+                       float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream);
+
+/* Linear interpolation of data at grid [N, ndim, nP], nP = prod(outsize).
+ * Replaces libcpab/pytorch/interpolation.py:11-171 (interpolate1D/2D/3D). */
+int cpab_interpolate_fwd(int ndim, const float* data, int N, int C, const int* dsize, const float* grid,
+                         const int* outsize, float* out, hipStream_t stream);
+
+/* Backward of cpab_interpolate_fwd: grad_data (nullable, zero-filled by the
+ * caller, accumulated atomically) and grad_grid (nullable).  The reference
+ * gets these from torch autograd through interpolation.py. */
+int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* dsize, const float* grid,
+                         const int* outsize, const float* grad_out, float* grad_data, float* grad_grid,
+                         hipStream_t stream);
+
+/* Fused uniform meshgrid -> closed-form integration -> interpolation
+ * (Cpab.transform_data, libcpab/cpab.py:304-331).  lin holds the meshgrid axis
+ * coordinates concatenated (outsize[0] + outsize[1] + ... floats; the values
+ * of libcpab/pytorch/functions.py:102-108).  saved (nullable,
+ * cpab_saved_floats) keeps the warped points and Jacobians for the backward. */
+int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                            const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                            float* saved, void* workspace, size_t ws_bytes, hipStream_t stream);
+
+/* Backward of cpab_transform_data_fwd: grad_data (nullable, zero-filled,
+ * [n_theta, C, dsize...]) and grad_theta [n_theta, d].  saved may be NULL
+ * (the trajectory is then recomputed). */
+int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                            const int* dsize, const int* outsize, const float* lin, float t_end, const float* grad_out,
+                            const float* saved, float* grad_data, float* grad_theta, void* workspace, size_t ws_bytes,
+                            hipStream_t stream);
+
+/* Cell index of each point [ndim, nP] -> idx [nP] (int32).  Replaces
+ * findcellidx of libcpab/pytorch/findcellidx.py:16-148 (numpy semantics of
+ * libcpab/numpy/findcellidx.py). */
+int cpab_findcell(int ndim, const int* nc, const float* points, int nP, int* idx, hipStream_t stream);
+
+/* Velocity field of one theta ([1, d]) at points [ndim, nP] -> v [ndim, nP].
+ * Replaces calc_vectorfield, libcpab/pytorch/functions.py:111-129. */
+int cpab_vectorfield(const cpab_tess_t* t, const float* theta, const float* points, int nP, float* v, void* workspace,
+                     size_t ws_bytes, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPAB_HIP_H */

@@ -1,0 +1,84 @@
+"""ctypes binding of libcpab_hip.so (the C ABI declared in include/cpab_hip.h).
+
+The library is built in-tree by libcpab_amd/build.py (``hipcc`` for gfx950)
+and loaded from libcpab_amd/_lib/.  There is no fallback: a missing or
+unloadable library raises, so a GPU call can never silently run elsewhere.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libcpab_hip.so")
+
+_c_int = ctypes.c_int
+_c_float = ctypes.c_float
+_c_size = ctypes.c_size_t
+_c_ptr = ctypes.c_void_p
+
+
+class CpabTess(ctypes.Structure):
+    """Mirror of cpab_tess_t."""
+    _fields_ = [("ndim", _c_int), ("nc", _c_int * 3), ("nC", _c_int), ("d", _c_int),
+                ("B", _c_ptr), ("planes", _c_ptr), ("nbr", _c_ptr)]
+
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "cpab_status_string": (ctypes.c_char_p, [_c_int]),
+    "cpab_abi_version": (_c_int, []),
+    "cpab_workspace_bytes": (_c_size, [ctypes.POINTER(CpabTess), _c_int, _c_int]),
+    "cpab_saved_floats": (_c_size, [_c_int, _c_int, _c_int]),
+    "cpab_theta_to_A": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_ptr]),
+    "cpab_transform_fwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
+                                    _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_bwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
+                                    _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_interpolate_fwd": (_c_int, [_c_int, _c_ptr, _c_int, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr]),
+    "cpab_interpolate_bwd": (_c_int, [_c_int, _c_ptr, _c_int, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                      _c_ptr, _c_ptr]),
+    "cpab_transform_data_fwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr, _c_ptr,
+                                         _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_data_bwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr, _c_ptr,
+                                         _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_findcell": (_c_int, [_c_int, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr]),
+    "cpab_vectorfield": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+
+
+class CpabError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CpabError("libcpab_hip.so is not built (expected at %s); run "
+                            "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        msg = lib().cpab_status_string(status).decode()
+        raise CpabError("libcpab_hip: %s (status %d)" % (msg, status))
+
+
+def int3(vals):
+    arr = (_c_int * 3)()
+    for i, v in enumerate(list(vals)[:3]):
+        arr[i] = int(v)
+    for i in range(len(vals), 3):
+        arr[i] = 1
+    return arr

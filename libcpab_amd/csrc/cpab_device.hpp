@@ -1,0 +1,686 @@
+// Device-side integrator for continuous piecewise-affine (CPA) velocity
+// fields on gfx950.
+//
+// Coordinates.  All integration runs in scaled coordinates q = x * nc
+// (x in [0,1]^D, nc = cells per axis), where the field of cell c is
+// q' = Mq q + bq with Mq = S M S^-1, bq = S b, S = diag(nc).  Facets are unit
+// planes n.q + e <= 0 (inside), stored per cell with the cell across them.
+//
+// In-cell flow.  For v = M q + b the exact flow is
+//     q(s) = q0 + sum_{k>=1} s^k / k! M^(k-1) v0 ,   v0 = M q0 + b,
+// an entire series.  It is evaluated on chunks with |M|_F h <= RHO = 1 and
+// KS = 10 terms, so the truncation (< RHO^10/11! relative to the chunk
+// displacement, 2.5e-8) is below FP32 resolution; no eigenvalue case analysis
+// (real / complex / repeated) and no transcendental calls are needed in 2D/3D.
+// 1D uses the closed form q(s) = q0 + s*phi1(a s)*v0 and the analytic hit time
+// s = (qb-q0)/v0 * log1p(r)/r, r = a (qb-q0)/v0.
+//
+// Boundary hits.  For each facet G(s) = n.q(s) + e is the same series; its
+// first root in the chunk is isolated by sampling G and G' at NSAMP points
+// (a local maximum with G > 0 inside a sample interval counts as a crossing)
+// and refined by Newton steps safeguarded by the bracket.
+//
+// Gradients.  The field is continuous across facets, so the sensitivity of the
+// final point obeys the variational equation with the saltation matrix
+// S = I + (v_next - v_prev) n^T / (n.v_prev) at every crossing (S = I for a
+// continuous field; it is applied anyway so the hit-time and cell-change terms
+// are part of the derivative).  Pass 1 carries J = dq(T)/dq0; the adjoint at
+// t = 0 is mu0 = J^T dL/dq(T).  Pass 2 re-integrates forward carrying
+// mu(s) = exp(-s M^T) mu0 and adds int_0^h mu(s) [q(s);1]^T ds to dL/dA_q of
+// the cell, an exact double series.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define CPAB_HD __host__ __device__
+#include <stdint.h>
+
+namespace cpab {
+
+constexpr int NF = 4;           // facet slots per cell
+constexpr int KS = 10;          // series terms
+constexpr float RHO = 1.0f;     // chunk bound |M|_F * h
+constexpr int NSAMP = 4;        // samples per chunk for root isolation
+constexpr int MAX_ITERS = 512;  // chunks + crossings per point (termination bound)
+constexpr float ZERO_HOP = 1e-7f;
+constexpr int MAX_ZERO_HOPS = 8;
+
+template <int D>
+struct CellTable {
+    const float* A;      // [nC, D, D+1]  q-space affine
+    const float* inv_n;  // [nC]          1 / |Mq|_F (inf for Mq = 0)
+    const float4* pl;    // [nC, NF]      (n0, n1, n2, e)
+    const int* nb;       // [nC, NF]      neighbour, <0: none
+};
+
+template <int D>
+struct PointState {
+    float q[D];
+    float T;        // remaining time
+    int c;          // current cell
+    int came_from;  // cell left by the last crossing
+    int zero_hops;  // consecutive crossings of (near) zero length
+    int iters;
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+template <int D>
+CPAB_HD __forceinline__ void load_affine(const CellTable<D>& t, int c, float (&M)[D][D], float (&b)[D]) {
+    const float* a = t.A + c * (D * (D + 1));
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) M[r][k] = a[r * (D + 1) + k];
+        b[r] = a[r * (D + 1) + D];
+    }
+}
+
+template <int D>
+CPAB_HD __forceinline__ void affine_apply(const float (&M)[D][D], const float (&b)[D], const float (&q)[D], float (&v)[D]) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        float acc = b[r];
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc = fmaf(M[r][k], q[k], acc);
+        v[r] = acc;
+    }
+}
+
+// G(s) and G'(s) of sum_k cf[k] s^k by simultaneous Horner.
+CPAB_HD __forceinline__ void horner2(const float (&cf)[KS + 1], float s, float& g, float& d) {
+    float gg = cf[KS], dd = 0.f;
+#pragma unroll
+    for (int k = KS - 1; k >= 0; --k) {
+        dd = fmaf(dd, s, gg);
+        gg = fmaf(gg, s, cf[k]);
+    }
+    g = gg;
+    d = dd;
+}
+
+CPAB_HD __forceinline__ float horner1(const float (&cf)[KS + 1], float s) {
+    float gg = cf[KS];
+#pragma unroll
+    for (int k = KS - 1; k >= 0; --k) gg = fmaf(gg, s, cf[k]);
+    return gg;
+}
+
+// Root of G in [lo, hi] with G(lo) <= 0 < G(hi): Newton safeguarded by bisection.
+CPAB_HD __forceinline__ float solve_bracket(const float (&cf)[KS + 1], float lo, float hi, float glo, float ghi) {
+    float s = lo + (hi - lo) * (glo / (glo - ghi));
+    if (!(s > lo && s < hi)) s = 0.5f * (lo + hi);
+    for (int it = 0; it < 24; ++it) {
+        float g, d;
+        horner2(cf, s, g, d);
+        if (g == 0.f) return s;
+        if (g > 0.f) hi = s; else lo = s;
+        float sn = s - g / d;
+        if (!(sn > lo && sn < hi)) sn = 0.5f * (lo + hi);
+        if (fabsf(sn - s) <= 1e-7f * hi || hi - lo <= 1e-7f * hi) { s = sn; break; }
+        s = sn;
+    }
+    return s;
+}
+
+// Zero of G' in [lo, hi] given G'(lo) and G'(hi) of opposite signs:
+// Illinois-modified secant steps inside the shrinking bracket; returns the
+// bracket end with the smaller |G'|.
+CPAB_HD __forceinline__ float crit_point(const float (&cf)[KS + 1], float lo, float hi, float dlo, float dhi) {
+    int side = 0;
+    for (int it = 0; it < 10; ++it) {
+        const float m = (lo * dhi - hi * dlo) / (dhi - dlo);
+        if (!(m > lo && m < hi)) break;
+        float gm, dm;
+        horner2(cf, m, gm, dm);
+        if (dm == 0.f) return m;
+        if ((dm > 0.f) == (dlo > 0.f)) {
+            lo = m;
+            dlo = dm;
+            if (side == -1) dhi *= 0.5f;
+            side = -1;
+        } else {
+            hi = m;
+            dhi = dm;
+            if (side == 1) dlo *= 0.5f;
+            side = 1;
+        }
+    }
+    return (fabsf(dlo) < fabsf(dhi)) ? lo : hi;
+}
+
+// First time in [0, h] at which the trajectory leaves through the facet with
+// coefficients cf (G(s) = sum_k cf[k] s^k, G <= 0 inside), or -1.
+//  * A start on the facet (G(0) >= -tol0) moving out exits at 0 when
+//    allow_start.
+//  * A start on the facet moving in first dips below 0; a later positive
+//    sample is bracketed from the dip's minimum (zero of G').
+//  * Any other start on the facet (start exit disallowed, or tangential) is
+//    ignored until G has gone negative.
+//  * Between two negative samples a local maximum above 0 is a crossing.
+CPAB_HD __forceinline__ float first_exit(const float (&cf)[KS + 1], float h, float tol0, float tol1, bool allow_start) {
+    const bool on = cf[0] >= -tol0;
+    if (on && allow_start) {
+        if (cf[1] > tol1) return 0.f;
+        if (fabsf(cf[1]) <= tol1 && cf[2] > 0.f) return 0.f;
+    }
+    bool skip = on && !(cf[1] < -tol1);
+    bool dip = on && !skip;
+    float ps = 0.f, pg = cf[0], pd = cf[1];
+#pragma unroll
+    for (int i = 1; i <= NSAMP; ++i) {
+        const float s = h * ((float)i * (1.0f / NSAMP));
+        float g, d;
+        horner2(cf, s, g, d);
+        if (skip) {
+            if (g <= 0.f) skip = false;
+        } else if (dip) {
+            if (g <= 0.f) {
+                dip = false;
+            } else if (pd < 0.f && d > 0.f) {
+                const float m = crit_point(cf, ps, s, pd, d);
+                const float gm = horner1(cf, m);
+                if (gm < 0.f) return solve_bracket(cf, m, s, gm, g);
+                skip = true;  // never went inside: treat as outside until G < 0
+                dip = false;
+            }
+        } else {
+            if (g > 0.f && pg <= 0.f) return solve_bracket(cf, ps, s, pg, g);
+            if (pd > 0.f && d < 0.f) {
+                const float m = crit_point(cf, ps, s, pd, d);
+                const float gm = horner1(cf, m);
+                if (gm > 0.f) return solve_bracket(cf, ps, m, pg, gm);
+            }
+        }
+        ps = s;
+        pg = g;
+        pd = d;
+    }
+    return -1.f;
+}
+
+// Series vectors W[k] = M^k v0 / (k+1)!, so q(s) = q0 + sum_k s^(k+1) W[k].
+template <int D>
+CPAB_HD __forceinline__ void series(const float (&M)[D][D], const float (&v0)[D], float (&W)[KS][D]) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) W[0][a] = v0[a];
+#pragma unroll
+    for (int k = 1; k < KS; ++k) {
+        const float inv = 1.0f / (float)(k + 1);
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            float acc = 0.f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc = fmaf(M[r][a], W[k - 1][a], acc);
+            W[k][r] = acc * inv;
+        }
+    }
+}
+
+template <int D>
+CPAB_HD __forceinline__ void series_eval(const float (&q0)[D], const float (&W)[KS][D], float s, float (&q)[D]) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        float acc = W[KS - 1][a];
+#pragma unroll
+        for (int k = KS - 2; k >= 0; --k) acc = fmaf(acc, s, W[k][a]);
+        q[a] = fmaf(acc, s, q0[a]);
+    }
+}
+
+// y = exp(s M) u by nested Horner: y = u + s M (u + s/2 M (u + ...)).
+template <int D>
+CPAB_HD __forceinline__ void expm_apply(const float (&M)[D][D], float s, const float (&u)[D], float (&y)[D]) {
+    float r[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) r[a] = u[a];
+#pragma unroll
+    for (int k = KS; k >= 1; --k) {
+        const float f = s / (float)k;
+        float t[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float acc = 0.f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc = fmaf(M[i][a], r[a], acc);
+            t[i] = fmaf(f, acc, u[i]);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) r[a] = t[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) y[a] = r[a];
+}
+
+// What one chunk did; handed to the pass callbacks.
+template <int D>
+struct Chunk {
+    float M[D][D];
+    float b[D];
+    float q0[D];
+    float W[KS][D];  // W[0] = v0
+    float h;         // time advanced
+    int cell;
+    int facet;       // crossed facet (-1: none)
+    int next;        // cell entered (== cell if no crossing)
+    float q1[D];     // end point
+};
+
+// ---------------------------------------------------------------------------
+// one chunk of integration, D = 2, 3
+// ---------------------------------------------------------------------------
+template <int D>
+CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab, float tol0, Chunk<D>& ch) {
+    const int c = st.c;
+    load_affine<D>(tab, c, ch.M, ch.b);
+    float v0[D];
+    affine_apply<D>(ch.M, ch.b, st.q, v0);
+    float h = fminf(st.T, RHO * tab.inv_n[c]);
+    series<D>(ch.M, v0, ch.W);
+    float vn2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) vn2 = fmaf(v0[a], v0[a], vn2);
+    const float vn = sqrtf(vn2);
+    const float reach = 1.7183f * vn * h + tol0;
+    const float tol1 = 1e-6f * vn;
+
+    float best = -1.f;
+    int bf = -1;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int nbf = tab.nb[c * NF + f];
+        if (nbf < 0) continue;
+        const float4 P = tab.pl[c * NF + f];
+        const float n[3] = {P.x, P.y, P.z};
+        float g0 = P.w;
+#pragma unroll
+        for (int a = 0; a < D; ++a) g0 = fmaf(n[a], st.q[a], g0);
+        if (g0 < -reach) continue;  // facet out of reach within this chunk
+        float cf[KS + 1];
+        cf[0] = g0;
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
+            float acc = 0.f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc = fmaf(n[a], ch.W[k][a], acc);
+            cf[k + 1] = acc;
+        }
+        const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
+        const float s = first_exit(cf, h, tol0, tol1, allow_start);
+        if (s >= 0.f && (bf < 0 || s < best)) {
+            best = s;
+            bf = f;
+        }
+    }
+    const float step = (bf >= 0) ? best : h;
+#pragma unroll
+    for (int a = 0; a < D; ++a) ch.q0[a] = st.q[a];
+    series_eval<D>(ch.q0, ch.W, step, ch.q1);
+    ch.h = step;
+    ch.cell = c;
+    ch.facet = bf;
+    ch.next = (bf >= 0) ? tab.nb[c * NF + bf] : c;
+#pragma unroll
+    for (int a = 0; a < D; ++a) st.q[a] = ch.q1[a];
+    st.T = (bf >= 0) ? fmaxf(st.T - step, 0.f) : st.T - step;
+    if (bf < 0 && st.T < 0.f) st.T = 0.f;
+    if (bf >= 0) {
+        st.zero_hops = (step <= ZERO_HOP) ? st.zero_hops + 1 : 0;
+        st.came_from = c;
+        st.c = ch.next;
+    }
+    st.iters++;
+}
+
+// ---------------------------------------------------------------------------
+// 1D: closed form
+// ---------------------------------------------------------------------------
+CPAB_HD __forceinline__ float phi1(float z) {  // (e^z - 1)/z
+    return (fabsf(z) < 1e-4f) ? fmaf(z, fmaf(z, 1.0f / 6.0f, 0.5f), 1.0f) : expm1f(z) / z;
+}
+CPAB_HD __forceinline__ float phi2(float z) {  // (e^z - 1 - z)/z^2
+    if (fabsf(z) < 0.5f) {
+        float acc = 1.0f / 3628800.0f;  // 1/10!
+        acc = fmaf(acc, z, 1.0f / 362880.0f);
+        acc = fmaf(acc, z, 1.0f / 40320.0f);
+        acc = fmaf(acc, z, 1.0f / 5040.0f);
+        acc = fmaf(acc, z, 1.0f / 720.0f);
+        acc = fmaf(acc, z, 1.0f / 120.0f);
+        acc = fmaf(acc, z, 1.0f / 24.0f);
+        acc = fmaf(acc, z, 1.0f / 6.0f);
+        acc = fmaf(acc, z, 0.5f);
+        return acc;
+    }
+    return (expm1f(z) - z) / (z * z);
+}
+CPAB_HD __forceinline__ float log1p_over(float r) {  // log1p(r)/r
+    return (fabsf(r) < 1e-4f) ? fmaf(r, fmaf(r, 1.0f / 3.0f, -0.5f), 1.0f) : log1pf(r) / r;
+}
+
+struct Chunk1 {
+    float a, b, q0, v0, h;
+    int cell, facet, next;
+    float q1;
+};
+
+CPAB_HD __forceinline__ void advance1(PointState<1>& st, const CellTable<1>& tab, int nx, Chunk1& ch) {
+    const int c = st.c;
+    const float a = tab.A[2 * c], b = tab.A[2 * c + 1];
+    const float q0 = st.q[0];
+    const float v0 = fmaf(a, q0, b);
+    float step = st.T;
+    int facet = -1, next = c;
+    if (v0 != 0.f) {
+        const bool right = v0 > 0.f;
+        const int nbc = right ? c + 1 : c - 1;
+        if (nbc >= 0 && nbc < nx) {
+            const float qb = right ? (float)(c + 1) : (float)c;
+            const float dq = qb - q0;
+            const float r = a * dq / v0;
+            if (1.0f + r > 0.f) {
+                float th = (dq / v0) * log1p_over(r);
+                th = fmaxf(th, 0.f);
+                bool back = (st.zero_hops > 0 && nbc == st.came_from && th <= ZERO_HOP);
+                if (th < st.T && !back) {
+                    step = th;
+                    facet = right ? 1 : 0;
+                    next = nbc;
+                }
+            }
+        }
+    }
+    ch.a = a; ch.b = b; ch.q0 = q0; ch.v0 = v0; ch.h = step;
+    ch.cell = c; ch.facet = facet; ch.next = next;
+    if (facet >= 0) {
+        ch.q1 = (facet == 1) ? (float)(c + 1) : (float)c;
+        st.T = fmaxf(st.T - step, 0.f);
+        st.zero_hops = (step <= ZERO_HOP) ? st.zero_hops + 1 : 0;
+        st.came_from = c;
+        st.c = next;
+    } else {
+        ch.q1 = fmaf(step * phi1(a * step), v0, q0);
+        st.T = 0.f;
+    }
+    st.q[0] = ch.q1;
+    st.iters++;
+}
+
+// ---------------------------------------------------------------------------
+// initial cell (the reference's lookup, libcpab/numpy/findcellidx.py, in FP32)
+// ---------------------------------------------------------------------------
+CPAB_HD __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+CPAB_HD __forceinline__ int find_cell1(float x, int nx) {
+    float f = floorf(x * (float)nx);
+    f = fminf(fmaxf(f, 0.f), (float)(nx - 1));
+    return (int)f;
+}
+
+CPAB_HD __forceinline__ int find_cell2(float x, float y, int nx, int ny) {
+    const float qx = x * (float)nx, qy = y * (float)ny;
+    const int i = clampi((int)floorf(fminf(fmaxf(qx, 0.f), (float)nx)), 0, nx - 1);
+    const int j = clampi((int)floorf(fminf(fmaxf(qy, 0.f), (float)ny)), 0, ny - 1);
+    const int base = 4 * (i + j * nx);
+    if (x <= 0.f) {
+        if (y <= 0.f && qy < qx) return base;
+        if (y >= 1.f && qy - (float)ny > -qx) return base + 2;
+        return base + 3;
+    }
+    if (x >= 1.f) {
+        if (y <= 0.f && -qy > qx - (float)nx) return base;
+        if (y >= 1.f && qy - (float)ny > qx - (float)nx) return base + 2;
+        return base + 1;
+    }
+    if (y <= 0.f) return base;
+    if (y >= 1.f) return base + 2;
+    const float fx = qx - (float)i, fy = qy - (float)j;
+    if (fx < fy) return base + ((1.f - fx < fy) ? 2 : 3);
+    return base + ((1.f - fx < fy) ? 1 : 0);
+}
+
+CPAB_HD __forceinline__ int find_cell3(float x, float y, float z, int nx, int ny, int nz) {
+    float p[3] = {x, y, z};
+    const int n[3] = {nx, ny, nz};
+    const bool outside = x < 0.f || x > 1.f || y < 0.f || y > 1.f || z < 0.f || z > 1.f;
+    if (outside) {
+        // libcpab/numpy/findcellidx.py:85-108: push the point onto the boundary
+        const float half = 0.5f;
+        float d[3] = {x - half, y - half, z - half};
+        const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+        const float push[3] = {(ax < ay && ax < az) ? half / nx : 0.f, (ay < ax && ax < az) ? half / ny : 0.f,
+                               (az < ax && ax < ay) ? half / nz : 0.f};
+        const float ab[3] = {ax, ay, az};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (ab[a] > half) d[a] = copysignf(half - push[a], d[a]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[a] = d[a] + half;
+    }
+    int ijk[3];
+    float loc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float q = fminf(fmaxf(p[a], 0.f), 1.f) * (float)n[a];
+        ijk[a] = clampi((int)floorf(q), 0, n[a] - 1);
+        loc[a] = q - (float)ijk[a];
+    }
+    int idx = 5 * (ijk[0] + ijk[1] * nx + ijk[2] * nx * ny);
+    float lx = loc[0], ly = loc[1], lz = loc[2];
+    if ((ijk[0] + ijk[1] + ijk[2]) & 1) {
+        const float t = lx;
+        lx = ly;
+        ly = 1.f - t;
+    }
+    if (-lx - ly + lz >= 0.f) idx += 1;
+    else if (lx + ly + lz - 2.f >= 0.f) idx += 2;
+    else if (-lx + ly - lz >= 0.f) idx += 3;
+    else if (lx - ly - lz >= 0.f) idx += 4;
+    return idx;
+}
+
+template <int D>
+CPAB_HD __forceinline__ int find_cell(const float (&x)[D], const int* nc) {
+    if constexpr (D == 1) return find_cell1(x[0], nc[0]);
+    else if constexpr (D == 2) return find_cell2(x[0], x[1], nc[0], nc[1]);
+    else return find_cell3(x[0], x[1], x[2], nc[0], nc[1], nc[2]);
+}
+
+// ---------------------------------------------------------------------------
+// per-point integration with optional Jacobian (pass 1)
+// ---------------------------------------------------------------------------
+template <int D>
+CPAB_HD __forceinline__ void saltation_fwd(const CellTable<D>& tab, const Chunk<D>& ch, float (&J)[D][D]) {
+    // J <- S J,  S = I + (v_next - v_prev) n^T / (n . v_prev)
+    float Mn[D][D], bn[D], vp[D], vn[D];
+    load_affine<D>(tab, ch.next, Mn, bn);
+    affine_apply<D>(ch.M, ch.b, ch.q1, vp);
+    affine_apply<D>(Mn, bn, ch.q1, vn);
+    const float4 P = tab.pl[ch.cell * NF + ch.facet];
+    const float n[3] = {P.x, P.y, P.z};
+    float nv = 0.f, vv = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        nv = fmaf(n[a], vp[a], nv);
+        vv = fmaf(vp[a], vp[a], vv);
+    }
+    if (fabsf(nv) <= 1e-6f * sqrtf(vv)) return;
+    const float inv = 1.0f / nv;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        float nj = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) nj = fmaf(n[a], J[a][j], nj);
+#pragma unroll
+        for (int r = 0; r < D; ++r) J[r][j] = fmaf((vn[r] - vp[r]) * inv, nj, J[r][j]);
+    }
+}
+
+template <int D>
+CPAB_HD __forceinline__ void saltation_adj(const CellTable<D>& tab, const Chunk<D>& ch, float (&mu)[D]) {
+    // mu <- S^-T mu = mu - n ((v_next - v_prev) . mu) / (n . v_next)
+    float Mn[D][D], bn[D], vp[D], vn[D];
+    load_affine<D>(tab, ch.next, Mn, bn);
+    affine_apply<D>(ch.M, ch.b, ch.q1, vp);
+    affine_apply<D>(Mn, bn, ch.q1, vn);
+    const float4 P = tab.pl[ch.cell * NF + ch.facet];
+    const float n[3] = {P.x, P.y, P.z};
+    float nv = 0.f, vv = 0.f, um = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        nv = fmaf(n[a], vn[a], nv);
+        vv = fmaf(vn[a], vn[a], vv);
+        um = fmaf(vn[a] - vp[a], mu[a], um);
+    }
+    if (fabsf(nv) <= 1e-6f * sqrtf(vv)) return;
+    const float f = um / nv;
+#pragma unroll
+    for (int a = 0; a < D; ++a) mu[a] = fmaf(-n[a], f, mu[a]);
+}
+
+template <int D, bool WITHJ>
+CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+                                          float (&J)[D][D]) {
+    PointState<D> st;
+#pragma unroll
+    for (int a = 0; a < D; ++a) st.q[a] = q[a];
+    st.T = T;
+    st.c = c0;
+    st.came_from = -1;
+    st.zero_hops = 0;
+    st.iters = 0;
+    if constexpr (WITHJ) {
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int k = 0; k < D; ++k) J[r][k] = (r == k) ? 1.f : 0.f;
+    }
+    while (st.T > 0.f && st.iters < MAX_ITERS) {
+        if constexpr (D == 1) {
+            Chunk1 ch;
+            advance1(st, tab, nx, ch);
+            if constexpr (WITHJ) {
+                J[0][0] *= expf(ch.a * ch.h);
+                if (ch.facet >= 0) {
+                    const float vn = fmaf(tab.A[2 * ch.next], ch.q1, tab.A[2 * ch.next + 1]);
+                    const float vp = fmaf(ch.a, ch.q1, ch.b);
+                    if (fabsf(vp) > 1e-6f * fabsf(vn) && vp != 0.f) J[0][0] *= vn / vp;
+                }
+            }
+        } else {
+            Chunk<D> ch;
+            advance<D>(st, tab, tol0, ch);
+            if constexpr (WITHJ) {
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    float u[D], y[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) u[a] = J[a][j];
+                    expm_apply<D>(ch.M, ch.h, u, y);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) J[a][j] = y[a];
+                }
+                if (ch.facet >= 0) saltation_fwd<D>(tab, ch, J);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) q[a] = st.q[a];
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: gradient contribution of one chunk
+// ---------------------------------------------------------------------------
+template <int D>
+CPAB_HD __forceinline__ void chunk_grad(const Chunk<D>& ch, float (&mu)[D], float (&G)[D * (D + 1)]) {
+    const float h = ch.h;
+    float hp[KS + 2];
+    hp[0] = 1.f;
+#pragma unroll
+    for (int r = 1; r < KS + 2; ++r) hp[r] = hp[r - 1] * h;
+    float m[D], mend[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        m[a] = mu[a];
+        mend[a] = 0.f;
+    }
+#pragma unroll
+    for (int p = 0; p < D * (D + 1); ++p) G[p] = 0.f;
+#pragma unroll
+    for (int j = 0; j <= KS; ++j) {
+        // Y_j = sum_{k=0}^{KS-j} h^(j+k+1)/(j+k+1) X_k,  X_0 = [q0;1], X_k = [W[k-1];0]
+        const float w0 = hp[j + 1] * (1.0f / (float)(j + 1));
+        float y[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) y[a] = w0 * ch.q0[a];
+#pragma unroll
+        for (int k = 1; k <= KS - j; ++k) {
+            const float wk = hp[j + k + 1] * (1.0f / (float)(j + k + 1));
+#pragma unroll
+            for (int a = 0; a < D; ++a) y[a] = fmaf(wk, ch.W[k - 1][a], y[a]);
+        }
+        const float sg = (j & 1) ? -1.f : 1.f;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            const float mr = sg * m[r];
+#pragma unroll
+            for (int a = 0; a < D; ++a) G[r * (D + 1) + a] = fmaf(mr, y[a], G[r * (D + 1) + a]);
+            G[r * (D + 1) + D] = fmaf(mr, w0, G[r * (D + 1) + D]);
+            mend[r] = fmaf(sg * hp[j], m[r], mend[r]);
+        }
+        if (j < KS) {
+            const float inv = 1.0f / (float)(j + 1);
+            float t[D];
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                float acc = 0.f;
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc = fmaf(ch.M[a][r], m[a], acc);
+                t[r] = acc * inv;
+            }
+#pragma unroll
+            for (int r = 0; r < D; ++r) m[r] = t[r];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) mu[a] = mend[a];
+}
+
+CPAB_HD __forceinline__ void chunk_grad1(const Chunk1& ch, float& mu, float (&G)[2]) {
+    const float z = -ch.a * ch.h;
+    G[0] = mu * fmaf(ch.h * ch.h, ch.b * phi2(z), ch.q0 * ch.h);
+    G[1] = mu * ch.h * phi1(z);
+    mu *= expf(z);
+}
+
+// ---------------------------------------------------------------------------
+// wave-level deterministic accumulation of per-lane cell contributions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Adds g (P values) of every lane with cell >= 0 into acc[cell*P + p] of this
+// wave's LDS region; lanes sharing a cell are summed by a butterfly first, the
+// leader of each group writes.  Must be reached by all lanes of the wave.
+template <int P>
+__device__ __forceinline__ void wave_accumulate(float* acc, int cell, const float (&g)[P]) {
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(cell >= 0);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int c0 = __shfl(cell, leader, 64);
+        const bool mine = (cell == c0);
+        const uint64_t grp = __ballot(mine);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            float v = wave_sum(mine ? g[p] : 0.f);
+            if (lane == leader) acc[c0 * P + p] += v;
+        }
+        pending &= ~grp;
+    }
+}
+
+}  // namespace cpab

@@ -1,0 +1,1046 @@
+// HIP kernels and the extern "C" boundary of the CPA integration path.
+//
+// Kernel map (one block = 256 threads = 4 waves of 64; a block owns one
+// theta and a tile of 256*ppt points, its theta's per-cell parameters and the
+// facet table are staged in LDS; block ids are remapped so that all tiles of a
+// theta land on the same XCD and share its L2):
+//   k_theta_to_A        A = B theta, converted to q-space        (per theta)
+//   k_transform         closed-form forward of explicit points    (transform_grid)
+//   k_transform_data    meshgrid -> closed-form -> interpolation  (transform_data)
+//   k_transform_bwd     adjoint of k_transform, dL/dA partials per tile (+ dL/dpoints)
+//   k_transform_data_bwd  adjoint of k_transform_data (+ dL/ddata scatter)
+//   k_interp_fwd / k_interp_bwd   linear interpolation of data at a grid
+//   k_reduce_theta      per-theta sum of the tile partials, then B^T (deterministic)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cpab_device.hpp"
+#include "../../include/cpab_hip.h"
+
+using namespace cpab;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int WAVES = BLOCK / 64;
+
+// ---------------------------------------------------------------------------
+// argument blocks
+// ---------------------------------------------------------------------------
+struct Geo {
+    int ndim, nc0, nc1, nc2, nC;
+    const float4* pl;
+    const int* nb;
+    float tol0;
+};
+
+struct Src {  // where points come from
+    const float* pts;  // explicit [ndim, nP] or [n_theta, ndim, nP]
+    int per_theta;
+    const float* lin;  // meshgrid: concatenated axis coordinates
+    int g0, g1, g2;    // meshgrid / output sizes per axis
+};
+
+struct Img {  // data for interpolation, NC[W(,H(,D))] layout
+    const float* data;
+    float* gdata;
+    int C, s0, s1, s2;
+};
+
+struct Launch {
+    int n_theta, nP, tiles, ppt;
+    float t_end;
+};
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    // blocks are dispatched round-robin over 8 XCDs; give each XCD a contiguous
+    // range of logical tiles so one theta's tiles share an L2
+    const int xcd = b & 7, k = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+template <int D>
+__device__ __forceinline__ int nc_at(const Geo& g, int a) {
+    return a == 0 ? g.nc0 : (a == 1 ? g.nc1 : g.nc2);
+}
+
+// point index p of output-order index t (last output axis fastest)
+template <int D>
+__device__ __forceinline__ int out_to_point(const Src& s, int t, int (&ix)[3]) {
+    if constexpr (D == 1) {
+        ix[0] = t;
+        return t;
+    } else if constexpr (D == 2) {
+        ix[1] = t % s.g1;
+        ix[0] = t / s.g1;
+        return ix[0] + s.g0 * ix[1];
+    } else {
+        ix[2] = t % s.g2;
+        const int r = t / s.g2;
+        ix[1] = r % s.g1;
+        ix[0] = r / s.g1;
+        return ix[0] + s.g0 * (ix[1] + s.g1 * ix[2]);
+    }
+}
+
+// stage theta's q-space affine parameters, their inverse norms and the facet table
+template <int D>
+__device__ __forceinline__ CellTable<D> stage_tables(float* smem, const float* Aq, const Geo& g) {
+    constexpr int P = D * (D + 1);
+    const int nC = g.nC;
+    float* sA = smem;
+    float* sInv = sA + nC * P;
+    float4* sPl = reinterpret_cast<float4*>(sInv + ((nC + 3) & ~3));
+    int* sNb = reinterpret_cast<int*>(sPl + (D > 1 ? nC * NF : 0));
+    for (int i = threadIdx.x; i < nC * P; i += BLOCK) sA[i] = Aq[i];
+    if constexpr (D > 1) {
+        for (int i = threadIdx.x; i < nC * NF; i += BLOCK) {
+            sPl[i] = g.pl[i];
+            sNb[i] = g.nb[i];
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nC; c += BLOCK) {
+        float s = 0.f;
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float m = sA[c * P + r * (D + 1) + k];
+                s = fmaf(m, m, s);
+            }
+        sInv[c] = s > 0.f ? rsqrtf(s) : INFINITY;
+    }
+    __syncthreads();
+    CellTable<D> t;
+    t.A = sA;
+    t.inv_n = sInv;
+    t.pl = sPl;
+    t.nb = sNb;
+    return t;
+}
+
+template <int D>
+__host__ __device__ constexpr size_t table_floats(int nC) {
+    return (size_t)nC * D * (D + 1) + ((nC + 3) & ~3) + (D > 1 ? (size_t)nC * NF * 4 + (size_t)nC * NF : 0);
+}
+
+// Pass 2 over one point per lane (wave-convergent); accumulates into acc.
+template <int D>
+__device__ __forceinline__ void pass2(bool valid, const float (&q0)[D], int c0, float T, float (&mu)[D],
+                                      const CellTable<D>& tab, const Geo& g, float* acc) {
+    constexpr int P = D * (D + 1);
+    PointState<D> st;
+#pragma unroll
+    for (int a = 0; a < D; ++a) st.q[a] = q0[a];
+    st.T = valid ? T : 0.f;
+    st.c = c0;
+    st.came_from = -1;
+    st.zero_hops = 0;
+    st.iters = 0;
+    while (true) {
+        const bool active = st.T > 0.f && st.iters < MAX_ITERS;
+        if (__ballot(active) == 0) break;
+        int cell = -1;
+        float G[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) G[p] = 0.f;
+        if (active) {
+            if constexpr (D == 1) {
+                Chunk1 ch;
+                advance1(st, tab, g.nc0, ch);
+                float G1[2];
+                chunk_grad1(ch, mu[0], G1);
+                G[0] = G1[0];
+                G[1] = G1[1];
+                cell = ch.cell;
+                if (ch.facet >= 0) {
+                    const float vn = fmaf(tab.A[2 * ch.next], ch.q1, tab.A[2 * ch.next + 1]);
+                    const float vp = fmaf(ch.a, ch.q1, ch.b);
+                    if (fabsf(vn) > 1e-6f * fabsf(vp) && vn != 0.f) mu[0] *= vp / vn;
+                }
+            } else {
+                Chunk<D> ch;
+                advance<D>(st, tab, g.tol0, ch);
+                chunk_grad<D>(ch, mu, G);
+                cell = ch.cell;
+                if (ch.facet >= 0) saltation_adj<D>(tab, ch, mu);
+            }
+        }
+        wave_accumulate<P>(acc, cell, G);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// interpolation helpers (libcpab/pytorch/interpolation.py semantics)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void axis_corners(float x, int size, int& i0, int& i1, float& fd) {
+    const float xs = x * (float)(size - 1);
+    float f0 = floorf(xs);
+    f0 = fminf(fmaxf(f0, -1.f), (float)size);
+    int a0 = (int)f0;
+    int a1 = a0 + 1;
+    a0 = clampi(a0, 0, size - 1);
+    a1 = clampi(a1, 0, size - 1);
+    i0 = a0;
+    i1 = a1;
+    fd = xs - (float)a0;
+}
+
+// value of channel c at point y (x-space)
+template <int D>
+__device__ __forceinline__ float interp_value(const float* __restrict__ dp, const Img& im, const int (&i0)[3],
+                                              const int (&i1)[3], const float (&fd)[3]) {
+    if constexpr (D == 1) {
+        const float c0 = dp[i0[0]], c1 = dp[i1[0]];
+        return c0 * (1.f - fd[0]) + c1 * fd[0];
+    } else if constexpr (D == 2) {
+        const int H = im.s1;
+        const float c00 = dp[i0[0] * H + i0[1]], c01 = dp[i0[0] * H + i1[1]];
+        const float c10 = dp[i1[0] * H + i0[1]], c11 = dp[i1[0] * H + i1[1]];
+        const float c0 = c00 * (1.f - fd[0]) + c10 * fd[0];
+        const float c1 = c01 * (1.f - fd[0]) + c11 * fd[0];
+        return c0 * (1.f - fd[1]) + c1 * fd[1];
+    } else {
+        const int H = im.s1, Z = im.s2;
+        auto at = [&](int x, int y, int z) { return dp[(x * H + y) * Z + z]; };
+        const float c000 = at(i0[0], i0[1], i0[2]), c001 = at(i0[0], i0[1], i1[2]);
+        const float c010 = at(i0[0], i1[1], i0[2]), c011 = at(i0[0], i1[1], i1[2]);
+        const float c100 = at(i1[0], i0[1], i0[2]), c101 = at(i1[0], i0[1], i1[2]);
+        const float c110 = at(i1[0], i1[1], i0[2]), c111 = at(i1[0], i1[1], i1[2]);
+        const float c00 = c000 * (1.f - fd[0]) + c100 * fd[0];
+        const float c01 = c001 * (1.f - fd[0]) + c101 * fd[0];
+        const float c10 = c010 * (1.f - fd[0]) + c110 * fd[0];
+        const float c11 = c011 * (1.f - fd[0]) + c111 * fd[0];
+        const float c0 = c00 * (1.f - fd[1]) + c10 * fd[1];
+        const float c1 = c01 * (1.f - fd[1]) + c11 * fd[1];
+        return c0 * (1.f - fd[2]) + c1 * fd[2];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void corners(const float (&y)[D], const Img& im, int (&i0)[3], int (&i1)[3], float (&fd)[3]) {
+    const int sz[3] = {im.s0, im.s1, im.s2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        i0[a] = 0;
+        i1[a] = 0;
+        fd[a] = 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) axis_corners(y[a], sz[a], i0[a], i1[a], fd[a]);
+}
+
+// backward of interp_value for channel data dp with upstream g: scatters into
+// gp (nullable) and returns d value / d fd[a] accumulated into dfd
+template <int D>
+__device__ __forceinline__ void interp_backward(const float* __restrict__ dp, float* gp, const Img& im, const int (&i0)[3],
+                                                const int (&i1)[3], const float (&fd)[3], float g, float (&dfd)[3]) {
+    if constexpr (D == 1) {
+        const float c0 = dp[i0[0]], c1 = dp[i1[0]];
+        if (gp) {
+            atomicAdd(gp + i0[0], g * (1.f - fd[0]));
+            atomicAdd(gp + i1[0], g * fd[0]);
+        }
+        dfd[0] = fmaf(g, c1 - c0, dfd[0]);
+    } else if constexpr (D == 2) {
+        const int H = im.s1;
+        const float c00 = dp[i0[0] * H + i0[1]], c01 = dp[i0[0] * H + i1[1]];
+        const float c10 = dp[i1[0] * H + i0[1]], c11 = dp[i1[0] * H + i1[1]];
+        const float wx0 = 1.f - fd[0], wx1 = fd[0], wy0 = 1.f - fd[1], wy1 = fd[1];
+        if (gp) {
+            atomicAdd(gp + i0[0] * H + i0[1], g * wx0 * wy0);
+            atomicAdd(gp + i1[0] * H + i0[1], g * wx1 * wy0);
+            atomicAdd(gp + i0[0] * H + i1[1], g * wx0 * wy1);
+            atomicAdd(gp + i1[0] * H + i1[1], g * wx1 * wy1);
+        }
+        const float c0 = c00 * wx0 + c10 * wx1;
+        const float c1 = c01 * wx0 + c11 * wx1;
+        dfd[0] = fmaf(g, (c10 - c00) * wy0 + (c11 - c01) * wy1, dfd[0]);
+        dfd[1] = fmaf(g, c1 - c0, dfd[1]);
+    } else {
+        const int H = im.s1, Z = im.s2;
+        auto idx = [&](int x, int y, int z) { return (x * H + y) * Z + z; };
+        const float w[3][2] = {{1.f - fd[0], fd[0]}, {1.f - fd[1], fd[1]}, {1.f - fd[2], fd[2]}};
+        const int ii[3][2] = {{i0[0], i1[0]}, {i0[1], i1[1]}, {i0[2], i1[2]}};
+        float cv[2][2][2];
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+            for (int by = 0; by < 2; ++by)
+#pragma unroll
+                for (int bz = 0; bz < 2; ++bz) {
+                    const int o = idx(ii[0][bx], ii[1][by], ii[2][bz]);
+                    cv[bx][by][bz] = dp[o];
+                    if (gp) atomicAdd(gp + o, g * w[0][bx] * w[1][by] * w[2][bz]);
+                }
+        // derivative of the trilinear blend w.r.t. fd[a]
+        float dx = 0.f, dy = 0.f, dz = 0.f;
+#pragma unroll
+        for (int by = 0; by < 2; ++by)
+#pragma unroll
+            for (int bz = 0; bz < 2; ++bz) dx += (cv[1][by][bz] - cv[0][by][bz]) * w[1][by] * w[2][bz];
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+            for (int bz = 0; bz < 2; ++bz) dy += (cv[bx][1][bz] - cv[bx][0][bz]) * w[0][bx] * w[2][bz];
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+            for (int by = 0; by < 2; ++by) dz += (cv[bx][by][1] - cv[bx][by][0]) * w[0][bx] * w[1][by];
+        dfd[0] = fmaf(g, dx, dfd[0]);
+        dfd[1] = fmaf(g, dy, dfd[1]);
+        dfd[2] = fmaf(g, dz, dfd[2]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// A = B theta in q-space: grid (ceil(Dt/256), n_theta)
+__global__ __launch_bounds__(BLOCK) void k_theta_to_A(const float* __restrict__ theta, const float* __restrict__ B,
+                                                      int d, int Dt, int ndim, int nc0, int nc1, int nc2,
+                                                      float* __restrict__ Aq) {
+    extern __shared__ float sth[];
+    const int th = blockIdx.y;
+    for (int k = threadIdx.x; k < d; k += BLOCK) sth[k] = theta[(size_t)th * d + k];
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= Dt) return;
+    const float* b = B + (size_t)i * d;
+    float acc = 0.f;
+    for (int k = 0; k < d; ++k) acc = fmaf(b[k], sth[k], acc);
+    const int P = ndim * (ndim + 1);
+    const int rem = i % P, r = rem / (ndim + 1), col = rem % (ndim + 1);
+    const int nc[3] = {nc0, nc1, nc2};
+    const float sc = (col < ndim) ? (float)nc[r] / (float)nc[col] : (float)nc[r];
+    Aq[(size_t)th * Dt + i] = acc * sc;
+}
+
+template <int D, bool WITHJ>
+__global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, const float* __restrict__ Aq,
+                                                     float* __restrict__ out, float* __restrict__ jac) {
+    extern __shared__ float smem[];
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const int nC = g.nC;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * D * (D + 1), g);
+    const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
+    for (int r = 0; r < L.ppt; ++r) {
+        const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
+        if (p >= L.nP) break;
+        float x[D], q[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            x[a] = pts[(size_t)a * L.nP + p];
+            q[a] = x[a] * (float)nc[a];
+        }
+        const int c0 = find_cell<D>(x, nc);
+        float J[D][D];
+        integrate<D, WITHJ>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+        float* o = out + (size_t)th * D * L.nP;
+#pragma unroll
+        for (int a = 0; a < D; ++a) o[(size_t)a * L.nP + p] = q[a] / (float)nc[a];
+        if constexpr (WITHJ) {
+            float* jo = jac + (size_t)th * D * D * L.nP;
+#pragma unroll
+            for (int r2 = 0; r2 < D; ++r2)
+#pragma unroll
+                for (int k = 0; k < D; ++k) jo[(size_t)(r2 * D + k) * L.nP + p] = J[r2][k];
+        }
+    }
+}
+
+// meshgrid -> integrate -> interpolate.  saved (nullable): [n_theta, D + D*D, nP]
+// holds the warped point (x-space) and the q-space Jacobian for the backward.
+template <int D, bool SAVE>
+__global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
+                                                          float* __restrict__ out, float* __restrict__ saved) {
+    extern __shared__ float smem[];
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const int nC = g.nC;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * D * (D + 1), g);
+    const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const int goff[3] = {0, s.g0, s.g0 + s.g1};
+    const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    for (int r = 0; r < L.ppt; ++r) {
+        const int t = (tile * L.ppt + r) * BLOCK + threadIdx.x;
+        if (t >= L.nP) break;
+        int ix[3];
+        const int p = out_to_point<D>(s, t, ix);
+        float x[D], q[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            x[a] = s.lin[goff[a] + ix[a]];
+            q[a] = x[a] * (float)nc[a];
+        }
+        const int c0 = find_cell<D>(x, nc);
+        float J[D][D];
+        integrate<D, SAVE>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+        float y[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) y[a] = q[a] / (float)nc[a];
+        if constexpr (SAVE) {
+            float* sv = saved + (size_t)th * (D + D * D) * L.nP;
+#pragma unroll
+            for (int a = 0; a < D; ++a) sv[(size_t)a * L.nP + p] = y[a];
+#pragma unroll
+            for (int r2 = 0; r2 < D; ++r2)
+#pragma unroll
+                for (int k = 0; k < D; ++k) sv[(size_t)(D + r2 * D + k) * L.nP + p] = J[r2][k];
+        }
+        int i0[3], i1[3];
+        float fd[3];
+        corners<D>(y, im, i0, i1, fd);
+        for (int c = 0; c < im.C; ++c) {
+            const float* dp = im.data + ((size_t)th * im.C + c) * dstride;
+            out[((size_t)th * im.C + c) * L.nP + t] = interp_value<D>(dp, im, i0, i1, fd);
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void write_partials(float* smem_acc, float* __restrict__ part, int nCP) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nCP; i += BLOCK) {
+        float s = 0.f;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s += smem_acc[w * nCP + i];
+        part[i] = s;
+    }
+}
+
+// adjoint of k_transform: grad_out [n_theta, D, nP] (x-space).  jac (nullable)
+// is the q-space Jacobian stored by the forward.  Writes per-tile partials of
+// dL/dA_q [n_theta, tiles, nC*P] and, if gpts != null, dL/dpoints.
+template <int D, bool HAVEJ>
+__global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s, const float* __restrict__ Aq,
+                                                         const float* __restrict__ gout, const float* __restrict__ jac,
+                                                         float* __restrict__ part, float* __restrict__ gpts) {
+    extern __shared__ float smem[];
+    constexpr int P = D * (D + 1);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const int nC = g.nC;
+    float* acc = smem + table_floats<D>(nC);
+    for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
+    float* wacc = acc + (threadIdx.x >> 6) * nC * P;
+    const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
+    for (int r = 0; r < L.ppt; ++r) {
+        const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
+        const bool valid = p < L.nP;
+        float q0[D], mu[D];
+        int c0 = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            q0[a] = 0.f;
+            mu[a] = 0.f;
+        }
+        if (valid) {
+            float x[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                x[a] = pts[(size_t)a * L.nP + p];
+                q0[a] = x[a] * (float)nc[a];
+            }
+            c0 = find_cell<D>(x, nc);
+            float J[D][D];
+            if constexpr (HAVEJ) {
+                const float* jo = jac + (size_t)th * D * D * L.nP;
+#pragma unroll
+                for (int r2 = 0; r2 < D; ++r2)
+#pragma unroll
+                    for (int k = 0; k < D; ++k) J[r2][k] = jo[(size_t)(r2 * D + k) * L.nP + p];
+            } else {
+                float q[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) q[a] = q0[a];
+                integrate<D, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+            }
+            float w[D];
+            const float* go = gout + (size_t)th * D * L.nP;
+#pragma unroll
+            for (int a = 0; a < D; ++a) w[a] = go[(size_t)a * L.nP + p] / (float)nc[a];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                float acc2 = 0.f;
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc2 = fmaf(J[a][k], w[a], acc2);
+                mu[k] = acc2;
+            }
+            if (gpts) {
+                float* gp = gpts + (size_t)th * D * L.nP;
+#pragma unroll
+                for (int a = 0; a < D; ++a) gp[(size_t)a * L.nP + p] = mu[a] * (float)nc[a];
+            }
+        }
+        pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc);
+    }
+    write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
+}
+
+// adjoint of k_transform_data.  gout [n_theta, C, outsize] in output order;
+// saved (nullable) from the forward.  im.gdata (nullable, zero-filled by the
+// caller) receives dL/ddata by atomic scatter.
+template <int D, bool HAVES>
+__global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
+                                                              const float* __restrict__ gout,
+                                                              const float* __restrict__ saved, float* __restrict__ part) {
+    extern __shared__ float smem[];
+    constexpr int P = D * (D + 1);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const int nC = g.nC;
+    float* acc = smem + table_floats<D>(nC);
+    for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
+    float* wacc = acc + (threadIdx.x >> 6) * nC * P;
+    const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const int goff[3] = {0, s.g0, s.g0 + s.g1};
+    const int sz[3] = {im.s0, im.s1, im.s2};
+    const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    for (int r = 0; r < L.ppt; ++r) {
+        const int t = (tile * L.ppt + r) * BLOCK + threadIdx.x;
+        const bool valid = t < L.nP;
+        float q0[D], mu[D];
+        int c0 = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            q0[a] = 0.f;
+            mu[a] = 0.f;
+        }
+        if (valid) {
+            int ix[3];
+            const int p = out_to_point<D>(s, t, ix);
+            float x[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                x[a] = s.lin[goff[a] + ix[a]];
+                q0[a] = x[a] * (float)nc[a];
+            }
+            c0 = find_cell<D>(x, nc);
+            float J[D][D], y[D];
+            if constexpr (HAVES) {
+                const float* sv = saved + (size_t)th * (D + D * D) * L.nP;
+#pragma unroll
+                for (int a = 0; a < D; ++a) y[a] = sv[(size_t)a * L.nP + p];
+#pragma unroll
+                for (int r2 = 0; r2 < D; ++r2)
+#pragma unroll
+                    for (int k = 0; k < D; ++k) J[r2][k] = sv[(size_t)(D + r2 * D + k) * L.nP + p];
+            } else {
+                float q[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) q[a] = q0[a];
+                integrate<D, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+#pragma unroll
+                for (int a = 0; a < D; ++a) y[a] = q[a] / (float)nc[a];
+            }
+            int i0[3], i1[3];
+            float fd[3], dfd[3] = {0.f, 0.f, 0.f};
+            corners<D>(y, im, i0, i1, fd);
+            for (int c = 0; c < im.C; ++c) {
+                const size_t base = ((size_t)th * im.C + c) * dstride;
+                const float gv = gout[((size_t)th * im.C + c) * L.nP + t];
+                interp_backward<D>(im.data + base, im.gdata ? im.gdata + base : nullptr, im, i0, i1, fd, gv, dfd);
+            }
+            float w[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) w[a] = dfd[a] * (float)(sz[a] - 1) / (float)nc[a];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                float acc2 = 0.f;
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc2 = fmaf(J[a][k], w[a], acc2);
+                mu[k] = acc2;
+            }
+        }
+        pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc);
+    }
+    write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
+}
+
+// interpolation at an explicit grid [N, D, nP]; out [N, C, outsize], output order
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_interp_fwd(int N, int nP, Src s, Img im, float* __restrict__ out) {
+    const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (gid >= (size_t)N * nP) return;
+    const int n = gid / nP, t = gid % nP;
+    int ix[3];
+    const int p = out_to_point<D>(s, t, ix);
+    float y[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) y[a] = s.pts[((size_t)n * D + a) * nP + p];
+    int i0[3], i1[3];
+    float fd[3];
+    corners<D>(y, im, i0, i1, fd);
+    const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    for (int c = 0; c < im.C; ++c)
+        out[((size_t)n * im.C + c) * nP + t] = interp_value<D>(im.data + ((size_t)n * im.C + c) * dstride, im, i0, i1, fd);
+}
+
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_interp_bwd(int N, int nP, Src s, Img im, const float* __restrict__ gout,
+                                                      float* __restrict__ ggrid) {
+    const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (gid >= (size_t)N * nP) return;
+    const int n = gid / nP, t = gid % nP;
+    int ix[3];
+    const int p = out_to_point<D>(s, t, ix);
+    float y[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) y[a] = s.pts[((size_t)n * D + a) * nP + p];
+    int i0[3], i1[3];
+    float fd[3], dfd[3] = {0.f, 0.f, 0.f};
+    corners<D>(y, im, i0, i1, fd);
+    const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    for (int c = 0; c < im.C; ++c) {
+        const size_t base = ((size_t)n * im.C + c) * dstride;
+        interp_backward<D>(im.data + base, im.gdata ? im.gdata + base : nullptr, im, i0, i1, fd,
+                           gout[((size_t)n * im.C + c) * nP + t], dfd);
+    }
+    if (ggrid) {
+        const int sz[3] = {im.s0, im.s1, im.s2};
+#pragma unroll
+        for (int a = 0; a < D; ++a) ggrid[((size_t)n * D + a) * nP + p] = dfd[a] * (float)(sz[a] - 1);
+    }
+}
+
+// per theta: dL/dA (x-space) = sum over tiles of the q-space partials, scaled
+// back; dL/dtheta = B^T dL/dA.  One block per theta, fixed summation order.
+__global__ __launch_bounds__(BLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
+                                                        int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
+                                                        float* __restrict__ gtheta, float* __restrict__ gA) {
+    extern __shared__ float sg[];
+    const int th = blockIdx.x;
+    const int P = ndim * (ndim + 1);
+    const int nc[3] = {nc0, nc1, nc2};
+    const float* pp = part + (size_t)th * tiles * Dt;
+    for (int i = threadIdx.x; i < Dt; i += BLOCK) {
+        float s = 0.f;
+        for (int t = 0; t < tiles; ++t) s += pp[(size_t)t * Dt + i];
+        const int rem = i % P, r = rem / (ndim + 1), col = rem % (ndim + 1);
+        const float sc = (col < ndim) ? (float)nc[r] / (float)nc[col] : (float)nc[r];
+        s *= sc;
+        sg[i] = s;
+        if (gA) gA[(size_t)th * Dt + i] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < d; k += BLOCK) {
+        float s = 0.f;
+        for (int i = 0; i < Dt; ++i) s = fmaf(B[(size_t)i * d + k], sg[i], s);
+        gtheta[(size_t)th * d + k] = s;
+    }
+}
+
+// cell index of each point (libcpab/numpy/findcellidx.py semantics, FP32)
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_findcell(int nc0, int nc1, int nc2, const float* __restrict__ pts, int nP,
+                                                    int* __restrict__ idx) {
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= nP) return;
+    const int nc[3] = {nc0, nc1, nc2};
+    float x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = pts[(size_t)a * nP + p];
+    idx[p] = find_cell<D>(x, nc);
+}
+
+// velocity of one theta's field at each point, x-space (libcpab/pytorch/functions.py:111-129)
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_vectorfield(Geo g, const float* __restrict__ Aq, const float* __restrict__ pts,
+                                                       int nP, float* __restrict__ v) {
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= nP) return;
+    const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    float x[D], q[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        x[a] = pts[(size_t)a * nP + p];
+        q[a] = x[a] * (float)nc[a];
+    }
+    const int c = find_cell<D>(x, nc);
+    const float* A = Aq + c * D * (D + 1);
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        float acc = A[r * (D + 1) + D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc = fmaf(A[r * (D + 1) + k], q[k], acc);
+        v[(size_t)r * nP + p] = acc / (float)nc[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+int cells_of(int ndim, const int* nc) {
+    int s = ndim == 1 ? 1 : (ndim == 2 ? 4 : 5);
+    for (int a = 0; a < ndim; ++a) s *= nc[a];
+    return s;
+}
+
+int check_tess(const cpab_tess_t* t) {
+    if (!t) return CPAB_ERR_ARG;
+    if (t->ndim < 1 || t->ndim > 3) return CPAB_ERR_NDIM;
+    for (int a = 0; a < t->ndim; ++a)
+        if (t->nc[a] < 1) return CPAB_ERR_ARG;
+    if (t->nC != cells_of(t->ndim, t->nc)) return CPAB_ERR_ARG;
+    if (t->ndim > 1 && (!t->planes || !t->nbr)) return CPAB_ERR_ARG;
+    return CPAB_OK;
+}
+
+Geo make_geo(const cpab_tess_t* t) {
+    Geo g;
+    g.ndim = t->ndim;
+    g.nc0 = t->nc[0];
+    g.nc1 = t->ndim > 1 ? t->nc[1] : 1;
+    g.nc2 = t->ndim > 2 ? t->nc[2] : 1;
+    g.nC = t->nC;
+    g.pl = reinterpret_cast<const float4*>(t->planes);
+    g.nb = t->nbr;
+    int m = g.nc0 > g.nc1 ? g.nc0 : g.nc1;
+    m = m > g.nc2 ? m : g.nc2;
+    g.tol0 = 2e-6f * (float)(m > 1 ? m : 1);
+    return g;
+}
+
+int pick_ppt(int nP) { return nP >= 4 * BLOCK * 16 ? 4 : (nP >= 2 * BLOCK * 16 ? 2 : 1); }
+
+size_t lds_tables(int ndim, int nC) {
+    switch (ndim) {
+        case 1: return table_floats<1>(nC) * 4;
+        case 2: return table_floats<2>(nC) * 4;
+        default: return table_floats<3>(nC) * 4;
+    }
+}
+
+size_t lds_bwd(int ndim, int nC) { return lds_tables(ndim, nC) + (size_t)WAVES * nC * ndim * (ndim + 1) * 4; }
+
+constexpr size_t LDS_LIMIT = 64 * 1024;
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int status_from_hip() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? CPAB_OK : CPAB_ERR_HIP;
+}
+
+int launch_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, float* Aq, hipStream_t st) {
+    const int Dt = t->nC * t->ndim * (t->ndim + 1);
+    dim3 grid((Dt + BLOCK - 1) / BLOCK, n_theta);
+    Geo g = make_geo(t);
+    hipLaunchKernelGGL(k_theta_to_A, grid, dim3(BLOCK), t->d * sizeof(float), st, theta, t->B, t->d, Dt, t->ndim, g.nc0,
+                       g.nc1, g.nc2, Aq);
+    return status_from_hip();
+}
+
+int launch_reduce(const cpab_tess_t* t, const float* part, int n_theta, int tiles, float* gtheta, float* gA,
+                  hipStream_t st) {
+    const int Dt = t->nC * t->ndim * (t->ndim + 1);
+    Geo g = make_geo(t);
+    hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(BLOCK), Dt * sizeof(float), st, part, tiles, Dt, t->ndim,
+                       g.nc0, g.nc1, g.nc2, t->B, t->d, gtheta, gA);
+    return status_from_hip();
+}
+
+}  // namespace
+
+// ===========================================================================
+// extern "C" boundary
+// ===========================================================================
+extern "C" {
+
+const char* cpab_status_string(int status) {
+    switch (status) {
+        case CPAB_OK: return "ok";
+        case CPAB_ERR_ARG: return "invalid argument";
+        case CPAB_ERR_NDIM: return "ndim must be 1, 2 or 3";
+        case CPAB_ERR_WORKSPACE: return "workspace too small";
+        case CPAB_ERR_HIP: return "HIP launch failed";
+        case CPAB_ERR_LDS: return "tessellation too large for the LDS budget of one block";
+        case CPAB_ERR_UNSUPPORTED: return "unsupported configuration";
+        default: return "unknown status";
+    }
+}
+
+int cpab_abi_version(void) { return CPAB_ABI_VERSION; }
+
+size_t cpab_workspace_bytes(const cpab_tess_t* t, int n_theta, int nP) {
+    if (check_tess(t) != CPAB_OK || n_theta < 0 || nP < 0) return 0;
+    const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
+    const int ppt = pick_ppt(nP);
+    size_t tiles = (nP + (size_t)BLOCK * ppt - 1) / ((size_t)BLOCK * ppt);
+    if (tiles == 0) tiles = 1;
+    return align256((size_t)n_theta * Dt * 4) + align256((size_t)n_theta * tiles * Dt * 4);
+}
+
+int cpab_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, float* Aq, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    if (!theta || !Aq || !t->B || n_theta < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0) return CPAB_OK;
+    return launch_theta_to_A(t, theta, n_theta, Aq, stream);
+}
+
+#define CPAB_DISPATCH_D(ndim, FN) \
+    switch (ndim) {               \
+        case 1: FN(1); break;     \
+        case 2: FN(2); break;     \
+        default: FN(3); break;    \
+    }
+
+int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
+                       int per_theta_grid, float t_end, float* out, float* jac, void* workspace, size_t ws_bytes,
+                       hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    if (!theta || !points || !out || n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0 || nP == 0) return CPAB_OK;
+    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    const size_t lds = lds_tables(t->ndim, t->nC);
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    float* Aq = static_cast<float*>(workspace);
+    if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(nP);
+    L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
+    L.t_end = t_end;
+    Geo g = make_geo(t);
+    Src s = {};
+    s.pts = points;
+    s.per_theta = per_theta_grid;
+    dim3 grid(L.tiles * n_theta);
+#define LAUNCH_TF(DD)                                                                                                \
+    if (jac)                                                                                                          \
+        hipLaunchKernelGGL((k_transform<DD, true>), grid, dim3(BLOCK), lds, stream, L, g, s, Aq, out, jac);           \
+    else                                                                                                              \
+        hipLaunchKernelGGL((k_transform<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, Aq, out, jac);
+    CPAB_DISPATCH_D(t->ndim, LAUNCH_TF)
+#undef LAUNCH_TF
+    return status_from_hip();
+}
+
+int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
+                       int per_theta_grid, float t_end, const float* grad_out, const float* jac, float* grad_theta,
+                       float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    if (!theta || !points || !grad_out || !grad_theta || n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0) return CPAB_OK;
+    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    const size_t lds = lds_bwd(t->ndim, t->nC);
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    const int Dt = t->nC * t->ndim * (t->ndim + 1);
+    float* Aq = static_cast<float*>(workspace);
+    float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) + align256((size_t)n_theta * Dt * 4));
+    if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(nP);
+    L.tiles = nP > 0 ? (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt) : 1;
+    L.t_end = t_end;
+    Geo g = make_geo(t);
+    Src s = {};
+    s.pts = points;
+    s.per_theta = per_theta_grid;
+    dim3 grid(L.tiles * n_theta);
+#define LAUNCH_TB(DD)                                                                                                   \
+    if (jac)                                                                                                             \
+        hipLaunchKernelGGL((k_transform_bwd<DD, true>), grid, dim3(BLOCK), lds, stream, L, g, s, Aq, grad_out, jac, part, \
+                           grad_points);                                                                                 \
+    else                                                                                                                 \
+        hipLaunchKernelGGL((k_transform_bwd<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, Aq, grad_out, jac,     \
+                           part, grad_points);
+    CPAB_DISPATCH_D(t->ndim, LAUNCH_TB)
+#undef LAUNCH_TB
+    if ((rc = status_from_hip())) return rc;
+    return launch_reduce(t, part, n_theta, L.tiles, grad_theta, nullptr, stream);
+}
+
+static int img_from(int ndim, const float* data, int C, const int* dsize, Img& im) {
+    if (!data || !dsize || C < 1) return CPAB_ERR_ARG;
+    im.data = data;
+    im.gdata = nullptr;
+    im.C = C;
+    im.s0 = dsize[0];
+    im.s1 = ndim > 1 ? dsize[1] : 1;
+    im.s2 = ndim > 2 ? dsize[2] : 1;
+    if (im.s0 < 1 || im.s1 < 1 || im.s2 < 1) return CPAB_ERR_ARG;
+    return CPAB_OK;
+}
+
+static int src_out(int ndim, const int* outsize, Src& s, int& nP) {
+    if (!outsize) return CPAB_ERR_ARG;
+    s.g0 = outsize[0];
+    s.g1 = ndim > 1 ? outsize[1] : 1;
+    s.g2 = ndim > 2 ? outsize[2] : 1;
+    if (s.g0 < 1 || s.g1 < 1 || s.g2 < 1) return CPAB_ERR_ARG;
+    nP = s.g0 * s.g1 * s.g2;
+    return CPAB_OK;
+}
+
+int cpab_interpolate_fwd(int ndim, const float* data, int N, int C, const int* dsize, const float* grid,
+                         const int* outsize, float* out, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    Img im;
+    Src s = {};
+    int nP, rc;
+    if ((rc = img_from(ndim, data, C, dsize, im))) return rc;
+    if ((rc = src_out(ndim, outsize, s, nP))) return rc;
+    if (!grid || !out || N < 0) return CPAB_ERR_ARG;
+    if (N == 0) return CPAB_OK;
+    s.pts = grid;
+    const size_t total = (size_t)N * nP;
+    dim3 g((unsigned)((total + BLOCK - 1) / BLOCK));
+#define LAUNCH_IF(DD) hipLaunchKernelGGL((k_interp_fwd<DD>), g, dim3(BLOCK), 0, stream, N, nP, s, im, out);
+    CPAB_DISPATCH_D(ndim, LAUNCH_IF)
+#undef LAUNCH_IF
+    return status_from_hip();
+}
+
+int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* dsize, const float* grid,
+                         const int* outsize, const float* grad_out, float* grad_data, float* grad_grid,
+                         hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    Img im;
+    Src s = {};
+    int nP, rc;
+    if ((rc = img_from(ndim, data, C, dsize, im))) return rc;
+    if ((rc = src_out(ndim, outsize, s, nP))) return rc;
+    if (!grid || !grad_out || N < 0) return CPAB_ERR_ARG;
+    if (N == 0) return CPAB_OK;
+    im.gdata = grad_data;
+    s.pts = grid;
+    const size_t total = (size_t)N * nP;
+    dim3 g((unsigned)((total + BLOCK - 1) / BLOCK));
+#define LAUNCH_IB(DD) hipLaunchKernelGGL((k_interp_bwd<DD>), g, dim3(BLOCK), 0, stream, N, nP, s, im, grad_out, grad_grid);
+    CPAB_DISPATCH_D(ndim, LAUNCH_IB)
+#undef LAUNCH_IB
+    return status_from_hip();
+}
+
+size_t cpab_saved_floats(int ndim, int n_theta, int nP) { return (size_t)n_theta * (ndim + ndim * ndim) * nP; }
+
+int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                            const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                            float* saved, void* workspace, size_t ws_bytes, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    Img im;
+    Src s = {};
+    int nP;
+    if ((rc = img_from(t->ndim, data, C, dsize, im))) return rc;
+    if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
+    if (!theta || !lin || !out || n_theta < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0) return CPAB_OK;
+    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    const size_t lds = lds_tables(t->ndim, t->nC);
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    float* Aq = static_cast<float*>(workspace);
+    if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
+    s.lin = lin;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(nP);
+    L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
+    L.t_end = t_end;
+    Geo g = make_geo(t);
+    dim3 grid(L.tiles * n_theta);
+#define LAUNCH_DF(DD)                                                                                                    \
+    if (saved)                                                                                                            \
+        hipLaunchKernelGGL((k_transform_data<DD, true>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, out, saved);    \
+    else                                                                                                                  \
+        hipLaunchKernelGGL((k_transform_data<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, out, saved);
+    CPAB_DISPATCH_D(t->ndim, LAUNCH_DF)
+#undef LAUNCH_DF
+    return status_from_hip();
+}
+
+int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                            const int* dsize, const int* outsize, const float* lin, float t_end, const float* grad_out,
+                            const float* saved, float* grad_data, float* grad_theta, void* workspace, size_t ws_bytes,
+                            hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    Img im;
+    Src s = {};
+    int nP;
+    if ((rc = img_from(t->ndim, data, C, dsize, im))) return rc;
+    if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
+    if (!theta || !lin || !grad_out || !grad_theta || n_theta < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0) return CPAB_OK;
+    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    const size_t lds = lds_bwd(t->ndim, t->nC);
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    const int Dt = t->nC * t->ndim * (t->ndim + 1);
+    float* Aq = static_cast<float*>(workspace);
+    float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) + align256((size_t)n_theta * Dt * 4));
+    if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
+    im.gdata = grad_data;
+    s.lin = lin;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(nP);
+    L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
+    L.t_end = t_end;
+    Geo g = make_geo(t);
+    dim3 grid(L.tiles * n_theta);
+#define LAUNCH_DB(DD)                                                                                                     \
+    if (saved)                                                                                                             \
+        hipLaunchKernelGGL((k_transform_data_bwd<DD, true>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, grad_out,    \
+                           saved, part);                                                                                   \
+    else                                                                                                                   \
+        hipLaunchKernelGGL((k_transform_data_bwd<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, grad_out,   \
+                           saved, part);
+    CPAB_DISPATCH_D(t->ndim, LAUNCH_DB)
+#undef LAUNCH_DB
+    if ((rc = status_from_hip())) return rc;
+    return launch_reduce(t, part, n_theta, L.tiles, grad_theta, nullptr, stream);
+}
+
+int cpab_findcell(int ndim, const int* nc, const float* points, int nP, int* idx, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    if (!nc || !points || !idx || nP < 0) return CPAB_ERR_ARG;
+    if (nP == 0) return CPAB_OK;
+    const int n0 = nc[0], n1 = ndim > 1 ? nc[1] : 1, n2 = ndim > 2 ? nc[2] : 1;
+    dim3 g((nP + BLOCK - 1) / BLOCK);
+#define LAUNCH_FC(DD) hipLaunchKernelGGL((k_findcell<DD>), g, dim3(BLOCK), 0, stream, n0, n1, n2, points, nP, idx);
+    CPAB_DISPATCH_D(ndim, LAUNCH_FC)
+#undef LAUNCH_FC
+    return status_from_hip();
+}
+
+int cpab_vectorfield(const cpab_tess_t* t, const float* theta, const float* points, int nP, float* v, void* workspace,
+                     size_t ws_bytes, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    if (!theta || !points || !v || nP < 0) return CPAB_ERR_ARG;
+    if (nP == 0) return CPAB_OK;
+    if (ws_bytes < cpab_workspace_bytes(t, 1, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    float* Aq = static_cast<float*>(workspace);
+    if ((rc = launch_theta_to_A(t, theta, 1, Aq, stream))) return rc;
+    Geo g = make_geo(t);
+    dim3 grid((nP + BLOCK - 1) / BLOCK);
+#define LAUNCH_VF(DD) hipLaunchKernelGGL((k_vectorfield<DD>), grid, dim3(BLOCK), 0, stream, g, Aq, points, nP, v);
+    CPAB_DISPATCH_D(t->ndim, LAUNCH_VF)
+#undef LAUNCH_VF
+    return status_from_hip();
+}
+
+}  // extern "C"

@@ -276,7 +276,7 @@ def interpolate(ndim, data, grid, outsize):
     for n in range(N):
         idx, w = _weights(ndim, grid[n], size)
         for ci, wi in zip(idx, w):
-            out[n] += data[(n, slice(None)) + tuple(ci)] * wi
+            out[n] += data[n][(slice(None),) + tuple(ci)] * wi
     return _to_outsize(out, outsize)
 
 
@@ -322,7 +322,7 @@ def interpolate_backward(ndim, data, grid, outsize, grad_out):
             wi = np.ones(grid.shape[-1])
             for a in range(ndim):
                 wi = wi * (cs[a][2] if bits[a] else 1 - cs[a][2])
-            vals = data[(n, slice(None)) + ci]  # [C, nP]
+            vals = data[n][(slice(None),) + ci]  # [C, nP]
             for c in range(C):
                 np.add.at(gdata[n, c], ci, go[n, c] * wi)
             dot = np.sum(vals * go[n], axis=0)

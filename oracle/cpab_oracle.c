@@ -452,18 +452,21 @@ static double solve_bracket(const double* c, double lo, double hi) {
     return s;
 }
 
-/* max of G on [lo, hi] where G' > 0 at lo and < 0 at hi */
-static double locate_max(const double* c, double lo, double hi) {
+/* zero of G' in (lo, hi) with G'(lo), G'(hi) of opposite signs (bisection) */
+static double crit_point(const double* c, double lo, double hi) {
+    const int up = dpoly(c, lo) > 0;
     for (int it = 0; it < 200 && hi - lo > 1e-17; ++it) {
         double m = 0.5 * (lo + hi);
-        if (dpoly(c, m) > 0) lo = m; else hi = m;
+        if ((dpoly(c, m) > 0) == up) lo = m; else hi = m;
     }
     return 0.5 * (lo + hi);
 }
 
 /* first exit time in [0, h] through a facet with coefficients c, or -1.
- * A point starting on the facet (G(0) >= -tol) and moving out exits at s = 0
- * when allow_start_exit; otherwise G is ignored until it has gone negative. */
+ * A start on the facet (G(0) >= -tol) moving out exits at s = 0 when
+ * allow_start_exit; a start on the facet moving in dips below 0 first, so a
+ * positive sample is bracketed from the dip minimum; any other start on the
+ * facet is ignored until G has gone negative. */
 static double first_exit(const double* c, double h, double vscale, int allow_start_exit) {
     const double tol0 = 1e-12, tol1 = 1e-13 * vscale;
     int on = c[0] >= -tol0;
@@ -471,17 +474,28 @@ static double first_exit(const double* c, double h, double vscale, int allow_sta
         if (c[1] > tol1) return 0.0;
         if (fabs(c[1]) <= tol1 && c[2] > 0) return 0.0;
     }
-    int skip = on;
+    int skip = on && !(c[1] < -tol1);
+    int dip = on && !skip;
     double prev_s = 0.0, prev_g = c[0], prev_d = dpoly(c, 0.0);
     for (int i = 1; i <= OR_NS; ++i) {
         double s = h * i / OR_NS;
         double gs = poly(c, s), ds = dpoly(c, s);
         if (skip) {
             if (gs <= 0) skip = 0;
+        } else if (dip) {
+            if (gs <= 0) {
+                dip = 0;
+            } else if (prev_d < 0 && ds > 0) {
+                double m = crit_point(c, prev_s, s);
+                double gm = poly(c, m);
+                if (gm < 0) return solve_bracket(c, m, s);
+                skip = 1;
+                dip = 0;
+            }
         } else {
             if (gs > 0 && prev_g <= 0) return solve_bracket(c, prev_s, s);
             if (prev_d > 0 && ds < 0) {
-                double m = locate_max(c, prev_s, s);
+                double m = crit_point(c, prev_s, s);
                 if (poly(c, m) > 0) return solve_bracket(c, prev_s, m);
             }
         }

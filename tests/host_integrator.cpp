@@ -1,0 +1,71 @@
+// Host build of the device integrator (libcpab_amd/csrc/cpab_device.hpp) for
+// CPU tests: the same per-point code path as k_transform, compiled for x86 so
+// its logic can be compared with the FP64 oracle without a GPU.  Test-only;
+// the library never loads this.
+#include <math.h>
+#include <stdlib.h>
+
+#include "../libcpab_amd/csrc/cpab_device.hpp"
+
+using namespace cpab;
+
+template <int D>
+static void run(const int* nc, int nC, const float* Aq, const float* planes, const int* nbr, const float* pts,
+                int nP, int n_theta, int per_theta, float t_end, float tol0, float* out, float* jac) {
+    constexpr int P = D * (D + 1);
+    float* inv = (float*)malloc(sizeof(float) * nC);
+    for (int t = 0; t < n_theta; ++t) {
+        const float* A = Aq + (size_t)t * nC * P;
+        for (int c = 0; c < nC; ++c) {
+            float s = 0.f;
+            for (int r = 0; r < D; ++r)
+                for (int k = 0; k < D; ++k) s += A[c * P + r * (D + 1) + k] * A[c * P + r * (D + 1) + k];
+            inv[c] = s > 0.f ? 1.0f / sqrtf(s) : INFINITY;
+        }
+        CellTable<D> tab;
+        tab.A = A;
+        tab.inv_n = inv;
+        tab.pl = reinterpret_cast<const float4*>(planes);
+        tab.nb = nbr;
+        const float* pp = pts + (per_theta ? (size_t)t * D * nP : 0);
+        for (int p = 0; p < nP; ++p) {
+            float x[D], q[D], J[D][D];
+            for (int a = 0; a < D; ++a) {
+                x[a] = pp[(size_t)a * nP + p];
+                q[a] = x[a] * (float)nc[a];
+            }
+            const int c0 = find_cell<D>(x, nc);
+            integrate<D, true>(q, c0, t_end, tab, nc[0], tol0, J);
+            for (int a = 0; a < D; ++a) out[((size_t)t * D + a) * nP + p] = q[a] / (float)nc[a];
+            if (jac)
+                for (int r = 0; r < D; ++r)
+                    for (int k = 0; k < D; ++k) jac[((size_t)t * D * D + r * D + k) * nP + p] = J[r][k];
+        }
+    }
+    free(inv);
+}
+
+extern "C" int host_transform(int ndim, const int* nc, int nC, const float* Aq, const float* planes, const int* nbr,
+                              const float* pts, int nP, int n_theta, int per_theta, float t_end, float tol0,
+                              float* out, float* jac) {
+    if (ndim == 1) run<1>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
+    else if (ndim == 2) run<2>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
+    else run<3>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
+    return 0;
+}
+
+extern "C" int host_findcell(int ndim, const int* nc, const float* pts, int nP, int* idx) {
+    for (int p = 0; p < nP; ++p) {
+        if (ndim == 1) {
+            float x[1] = {pts[p]};
+            idx[p] = find_cell<1>(x, nc);
+        } else if (ndim == 2) {
+            float x[2] = {pts[p], pts[nP + p]};
+            idx[p] = find_cell<2>(x, nc);
+        } else {
+            float x[3] = {pts[p], pts[nP + p], pts[2 * nP + p]};
+            idx[p] = find_cell<3>(x, nc);
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,177 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Bars: warped points within 1e-5 of the exact FP64 flow; gradients within
+1e-3 relative (of the gradient's max magnitude) of the exact FP64 gradient;
+interpolation within 1e-5 of the FP64 restatement at the same grid.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from conftest import make_cpab
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-5
+GRAD_RTOL = 1e-3
+
+CASES = [  # tessellation, n_theta, theta scale
+    ("1d_10", 3, 0.8),
+    ("1d_50", 3, 0.5),
+    ("2d_3x3", 3, 0.5),
+    ("2d_4x4", 3, 0.3),
+    ("2d_2x2_nzb", 3, 0.5),
+    ("3d_2x2x2", 2, 0.3),
+]
+
+
+def _theta(d, n, scale, seed):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.randn(n, d, generator=g) * scale).float()
+
+
+def _points(ndim, nP, seed, lo=0.0, hi=1.0):
+    g = torch.Generator().manual_seed(seed + 17)
+    return (torch.rand(ndim, nP, generator=g) * (hi - lo) + lo).float()
+
+
+def _A64(B, theta, ndim):
+    return O.expand_A(B, theta.double().numpy(), ndim)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-12)
+
+
+@pytest.mark.parametrize("name,n_theta,scale", CASES)
+def test_transform_grid_matches_exact(cuda, name, n_theta, scale):
+    T, b = make_cpab(name)
+    ndim = len(b["nc"])
+    theta = _theta(T.get_theta_dim(), n_theta, scale, 1)
+    pts = _points(ndim, 2000, 2)
+    # include a meshgrid: its points sit on facets and on the domain boundary
+    side = {1: 101, 2: 23, 3: 9}[ndim]
+    mesh = T.uniform_meshgrid([side] * ndim).cpu()
+    grid = torch.cat([pts, mesh], dim=1)
+    out = T.transform_grid(grid.to(cuda), theta.to(cuda)).cpu().numpy()
+    ref = O.cf_forward(ndim, list(b["nc"]), _A64(b["B"], theta, ndim), b["planes"], b["nbr"], grid.double().numpy())
+    err = np.abs(out - ref).max()
+    print(name, "max |gpu - exact| =", err)
+    assert err <= POS_TOL
+
+
+@pytest.mark.parametrize("name,n_theta,scale", CASES)
+def test_transform_grid_per_theta_grid(cuda, name, n_theta, scale):
+    T, b = make_cpab(name)
+    ndim = len(b["nc"])
+    theta = _theta(T.get_theta_dim(), n_theta, scale, 3)
+    grid = torch.stack([_points(ndim, 500, 10 + i) for i in range(n_theta)])
+    out = T.transform_grid(grid.to(cuda), theta.to(cuda)).cpu().numpy()
+    ref = O.cf_forward(ndim, list(b["nc"]), _A64(b["B"], theta, ndim), b["planes"], b["nbr"], grid.double().numpy())
+    assert np.abs(out - ref).max() <= POS_TOL
+
+
+@pytest.mark.parametrize("name,n_theta,scale", CASES)
+@pytest.mark.parametrize("store", [True, False])
+def test_transform_grid_gradients(cuda, name, n_theta, scale, store):
+    T, b = make_cpab(name)
+    T.set_solver_params(store_jacobian=store)
+    ndim = len(b["nc"])
+    theta = _theta(T.get_theta_dim(), n_theta, scale, 4)
+    pts = _points(ndim, 400, 5)
+    g = torch.Generator().manual_seed(6)
+    w = torch.randn(n_theta, ndim, pts.shape[1], generator=g)
+    th = theta.to(cuda).requires_grad_(True)
+    pp = pts.to(cuda).requires_grad_(True)
+    out = T.transform_grid(pp, th)
+    (out * w.to(cuda)).sum().backward()
+    A = _A64(b["B"], theta, ndim)
+    gA, gp = O.cf_backward(ndim, list(b["nc"]), A, b["planes"], b["nbr"], pts.double().numpy(), w.double().numpy())
+    gtheta = gA.reshape(n_theta, -1) @ b["B"]
+    rt = _rel(th.grad.cpu().double().numpy(), gtheta)
+    rp = _rel(pp.grad.cpu().double().numpy(), gp.sum(0))
+    print(name, "grad theta rel err", rt, "grad points rel err", rp)
+    assert rt <= GRAD_RTOL
+    assert rp <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("ndim,size,outsize", [(1, [40], [57]), (2, [13, 17], [11, 19]), (3, [7, 6, 5], [5, 8, 6])])
+def test_interpolate_matches_restatement(cuda, ndim, size, outsize):
+    g = torch.Generator().manual_seed(7)
+    N, C = 3, 2
+    data = torch.rand((N, C) + tuple(size), generator=g)
+    nP = int(np.prod(outsize))
+    grid = torch.rand(N, ndim, nP, generator=g) * 1.2 - 0.1  # includes clamped corners
+    gout = torch.randn((N, C) + tuple(outsize), generator=g)
+    dd = data.to(cuda).requires_grad_(True)
+    gg = grid.to(cuda).requires_grad_(True)
+    from libcpab_amd.pytorch.functions import interpolate
+    out = interpolate(ndim, dd, gg, outsize)
+    (out * gout.to(cuda)).sum().backward()
+    ref = O.interpolate(ndim, data.double().numpy(), grid.double().numpy(), outsize)
+    gd, ggr = O.interpolate_backward(ndim, data.double().numpy(), grid.double().numpy(), outsize, gout.double().numpy())
+    assert np.abs(out.detach().cpu().numpy() - ref).max() <= 1e-5
+    assert np.abs(dd.grad.cpu().numpy() - gd).max() <= 1e-4
+    assert _rel(gg.grad.cpu().double().numpy(), ggr) <= 1e-4
+
+
+@pytest.mark.parametrize("name,n_theta,scale,size", [("1d_10", 3, 0.8, [64]), ("2d_3x3", 3, 0.5, [24, 20]),
+                                                      ("3d_2x2x2", 2, 0.3, [9, 8, 7])])
+@pytest.mark.parametrize("store", [True, False])
+def test_transform_data_fused_matches_unfused(cuda, name, n_theta, scale, size, store):
+    """Fused meshgrid->integrate->interpolate equals transform_grid + interpolate,
+    forward and backward, and the warped grid equals the exact flow."""
+    T, b = make_cpab(name)
+    T.set_solver_params(store_jacobian=store)
+    ndim = len(b["nc"])
+    theta = _theta(T.get_theta_dim(), n_theta, scale, 8)
+    g = torch.Generator().manual_seed(9)
+    data = torch.rand((n_theta, 2) + tuple(size), generator=g)
+    gout = torch.randn((n_theta, 2) + tuple(size), generator=g)
+    th1 = theta.to(cuda).requires_grad_(True)
+    d1 = data.to(cuda).requires_grad_(True)
+    out1 = T.transform_data(d1, th1, size)
+    (out1 * gout.to(cuda)).sum().backward()
+    th2 = theta.to(cuda).requires_grad_(True)
+    d2 = data.to(cuda).requires_grad_(True)
+    grid = T.uniform_meshgrid(size)
+    gt = T.transform_grid(grid, th2)
+    out2 = T.interpolate(d2, gt, size)
+    (out2 * gout.to(cuda)).sum().backward()
+    assert torch.allclose(out1, out2, atol=1e-6, rtol=0)
+    assert torch.allclose(d1.grad, d2.grad, atol=1e-5, rtol=0)
+    assert _rel(th1.grad.cpu().double().numpy(), th2.grad.cpu().double().numpy()) <= 1e-4
+    # against the exact flow + FP64 interpolation at the exact grid
+    ref_grid = O.cf_forward(ndim, list(b["nc"]), _A64(b["B"], theta, ndim), b["planes"], b["nbr"],
+                            grid.cpu().double().numpy())
+    assert np.abs(gt.detach().cpu().numpy() - ref_grid).max() <= POS_TOL
+    ref = O.interpolate(ndim, data.double().numpy(), ref_grid, size)
+    # interpolation amplifies grid error by at most |grad data| ~ (size-1)
+    assert np.abs(out1.detach().cpu().numpy() - ref).max() <= POS_TOL * max(size) * 2 + 1e-6
+
+
+def test_identity_theta_is_exact_identity(cuda):
+    T, b = make_cpab("2d_4x4")
+    grid = T.uniform_meshgrid([64, 48])
+    th = T.identity(4)
+    out = T.transform_grid(grid, th)
+    assert torch.equal(out, grid[None].expand(4, -1, -1))
+    data = torch.rand(4, 3, 64, 48, device=cuda)
+    # x * (W-1) of a float32 linspace value is within an ulp of the pixel
+    # index, so the identity reproduces the data to ~1e-6
+    assert torch.allclose(T.transform_data(data, th, [64, 48]), data, atol=1e-5, rtol=0)
+
+
+def test_findcell_and_vectorfield(cuda):
+    T, b = make_cpab("2d_3x3")
+    pts = _points(2, 3000, 11, -0.2, 1.2)
+    idx = T.backend.findcellidx(2, pts.to(cuda), [3, 3]).cpu().numpy()
+    ref = O.findcell(2, [3, 3], pts.double().numpy())
+    # cell ids agree except for points within 1e-6 of a cell boundary
+    assert (idx != ref).mean() < 1e-3
+    theta = _theta(T.get_theta_dim(), 1, 0.5, 12)
+    v = T.calc_vectorfield(pts.to(cuda), theta.to(cuda)).cpu().numpy()
+    vr = O.vectorfield(2, [3, 3], _A64(b["B"], theta, 2)[0], pts.double().numpy())
+    same = idx == ref
+    assert np.abs(v[:, same] - vr[:, same]).max() <= 1e-5

@@ -1,0 +1,384 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+series term, a wrong neighbour or facet sign, a transposed adjoint, a missing
+saltation term) fails it:
+  * the restated fixed-step integrator against the reference's own compiled
+    op (golden fixtures; plus the live compiled op where /root/reference is
+    mounted): bit-exact,
+  * the exact flow against closed forms: the 1D piecewise-exponential
+    solution, and expm(A~) for globally affine fields (scipy),
+  * flow invariants: identity at theta = 0, semigroup, inverse,
+  * the reference's scheme converging to the exact flow as nstep grows,
+  * cell sequences against brute-force fine-step integration,
+  * the exact gradient against FP64 central finite differences, including a
+    discontinuous field where only the saltation (hit-time) terms make it
+    right,
+  * interpolation against torch.nn.functional.grid_sample and finite
+    differences.
+"""
+import numpy as np
+import pytest
+import torch
+from scipy.linalg import expm
+
+import oracle as O
+from conftest import load_basis, load_npz
+from libcpab_amd.core.tessellation import make_tessellation
+
+RNG = np.random.default_rng(12345)
+
+
+def _setup(name, n_theta, scale, seed=0):
+    b = load_basis(name)
+    nc = [int(v) for v in b["nc"]]
+    ndim = len(nc)
+    rng = np.random.default_rng(seed)
+    theta = rng.standard_normal((n_theta, b["B"].shape[1])) * scale
+    return b, nc, ndim, O.expand_A(b["B"], theta, ndim), theta
+
+
+# --------------------------------------------------------------------------
+# reference restatement
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "3d_2x2x2"])
+def test_step_restatement_matches_reference_golden(name):
+    f = load_npz("ref_step_%s.npz" % name)
+    nc = [int(v) for v in f["nc"]]
+    ndim = len(nc)
+    fwd = O.step_forward_f32(ndim, nc, int(f["nstep"]), f["points"], f["trels"])
+    assert np.array_equal(fwd, f["forward"])
+    bwd = O.step_backward_f32(ndim, nc, int(f["nstep"]), f["points"], f["As"], f["Bs"])
+    assert np.array_equal(bwd, f["backward"])
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference sources not mounted (oracle/_ref not built)")
+@pytest.mark.parametrize("name", ["1d_50", "2d_4x4", "2d_2x2_nzb", "3d_2x2x2"])
+def test_step_restatement_matches_compiled_reference(name):
+    b, nc, ndim, A, theta = _setup(name, 2, 0.6, seed=3)
+    pts = RNG.uniform(-0.1, 1.1, size=(ndim, 300)).astype(np.float32)
+    tr = O.trels(A, 50).astype(np.float32)
+    assert np.array_equal(O.ref_forward(ndim, nc, 50, pts, tr), O.step_forward_f32(ndim, nc, 50, pts, tr))
+    Bs = O.bases_Bs(b["B"].astype(np.float32), A.shape[1], ndim)
+    A32 = A.astype(np.float32)
+    assert np.array_equal(O.ref_backward(ndim, nc, 50, pts[:, :15], A32, Bs),
+                          O.step_backward_f32(ndim, nc, 50, pts[:, :15], A32, Bs))
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference sources not mounted (oracle/_ref not built)")
+@pytest.mark.parametrize("nc", [[7], [3, 3], [4, 2], [2, 3, 2]])
+def test_lookup_restatement_matches_compiled_reference(nc):
+    import ctypes
+    ndim = len(nc)
+    pts = RNG.uniform(-0.3, 1.3, size=(ndim, 3000)).astype(np.float32)
+    mine = O.findcell_f32(ndim, nc, pts)
+    arr = np.array(nc + [1] * (3 - ndim), dtype=np.int32)
+    ref = np.array([O.ref_lib().ref_findcellidx(ndim, np.ascontiguousarray(pts[:, i]).ctypes.data_as(ctypes.c_void_p),
+                                                arr.ctypes.data_as(ctypes.c_void_p)) for i in range(pts.shape[1])])
+    assert np.array_equal(mine, ref)
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nc,zb,d", [([10], True, 9), ([50], True, 49), ([3, 3], True, 34), ([4, 4], True, 62),
+                                     ([2, 2], False, 20), ([2, 2, 2], True, 27)])
+def test_basis_dimension_and_null_space(nc, zb, d):
+    t = make_tessellation(nc, zero_boundary=zb)
+    assert t.B.shape[1] == d
+    assert np.abs(t.L @ t.B).max() < 1e-9
+    assert np.allclose(t.B.T @ t.B, np.eye(d), atol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_4x4", "2d_2x2_nzb", "3d_2x2x2"])
+def test_fixture_basis_spans_the_rebuilt_null_space(name):
+    b = load_basis(name)
+    t = make_tessellation([int(v) for v in b["nc"]], zero_boundary=bool(b["zero_boundary"]))
+    P1 = b["B"] @ b["B"].T
+    P2 = t.B @ t.B.T
+    assert np.abs(P1 - P2).max() < 1e-9
+    planes, nbr = t.facet_table()
+    assert np.array_equal(nbr, b["nbr"]) and np.allclose(planes, b["planes"])
+
+
+@pytest.mark.parametrize("nc,zb", [([6], True), ([3, 3], True), ([2, 4], False), ([2, 2, 2], True)])
+def test_facet_table_agrees_with_lookup(nc, zb):
+    """Every point lies inside the facet region of the cell the reference's
+    lookup assigns it (2D/1D also outside the domain), and neighbours are
+    symmetric."""
+    t = make_tessellation(nc, zero_boundary=zb)
+    planes, nbr = t.facet_table()
+    ndim = len(nc)
+    lo, hi = (-0.4, 1.4) if ndim < 3 else (0.0, 1.0)
+    pts = RNG.uniform(lo, hi, size=(ndim, 20000))
+    c = O.findcell(ndim, nc, pts)
+    q = pts.T * np.array(nc)
+    G = np.einsum("pfa,pa->pf", planes[c, :, :ndim], q) + planes[c, :, 3]
+    G[nbr[c] == -2] = -1
+    assert G.max() < 1e-9
+    for ci in range(nbr.shape[0]):
+        for f in range(nbr.shape[1]):
+            if nbr[ci, f] >= 0:
+                assert ci in nbr[nbr[ci, f]]
+
+
+# --------------------------------------------------------------------------
+# exact flow
+# --------------------------------------------------------------------------
+def _flow1d_analytic(x, A, nx, T=1.0):
+    """Piecewise exact 1D flow from the closed form x(t) = (x0 + b/a) e^{at} - b/a
+    and hit times t = log(v(xb)/v(x0))/a, written independently of the oracle."""
+    c = min(max(int(np.floor(x * nx)), 0), nx - 1)
+    rem = T
+    for _ in range(10000):
+        a, b = A[c]
+        v = a * x + b
+        if v == 0:
+            return x
+        right = v > 0
+        xb = (c + 1) / nx if right else c / nx
+        at_end = (right and c == nx - 1) or (not right and c == 0)
+        th = np.inf
+        if not at_end:
+            vb = a * xb + b
+            if vb / v > 0:
+                th = np.log(vb / v) / a if a != 0 else (xb - x) / v
+        if th >= rem:
+            return (x + b / a) * np.exp(a * rem) - b / a if a != 0 else x + b * rem
+        x, rem, c = xb, rem - th, c + (1 if right else -1)
+    raise AssertionError("no convergence")
+
+
+@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("1d_50", 1.5)])
+def test_exact_1d_matches_closed_form(name, scale):
+    b, nc, ndim, A, theta = _setup(name, 3, scale, seed=5)
+    pts = np.concatenate([RNG.uniform(0, 1, 400), np.linspace(0, 1, 51)])[None]
+    out = O.cf_forward(1, nc, A, b["planes"], b["nbr"], pts)
+    ref = np.array([[_flow1d_analytic(x, A[k, :, 0, :], nc[0]) for x in pts[0]] for k in range(A.shape[0])])
+    assert np.abs(out[:, 0] - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("nc", [[4], [3, 3], [2, 2], [2, 2, 2]])
+def test_exact_affine_field_matches_expm(nc):
+    """A globally affine field lies in span(B) without the zero-boundary
+    constraint; its flow is expm(A~) for every point."""
+    ndim = len(nc)
+    zb = ndim == 3  # 3D: use a field with a fixed point at the centre so points stay inside
+    t = make_tessellation(nc, zero_boundary=False) if ndim < 3 else make_tessellation(nc, zero_boundary=True)
+    planes, nbr = t.facet_table()
+    rng = np.random.default_rng(7)
+    M = rng.standard_normal((ndim, ndim)) * 0.7
+    bvec = rng.standard_normal(ndim) * 0.3
+    if zb:
+        # a rotation-free linear field around the centre does not satisfy zero
+        # normal velocity; use an unconstrained basis for 3D instead
+        t = make_tessellation(nc, zero_boundary=True)
+    Avec = np.tile(np.concatenate([M, bvec[:, None]], 1).ravel(), t.nC)
+    if ndim == 3:
+        pytest.skip("3D affine-field check uses test_exact_affine_3d")
+    theta = t.B.T @ Avec
+    A = O.expand_A(t.B, theta[None], ndim)
+    assert np.abs(A.ravel() - Avec).max() < 1e-10
+    pts = rng.uniform(-0.2, 1.2, size=(ndim, 500))
+    out = O.cf_forward(ndim, nc, A, planes, nbr, pts)
+    Sq = np.zeros((ndim + 1, ndim + 1))
+    Sq[:ndim] = np.concatenate([M, bvec[:, None]], 1)
+    ref = (expm(Sq) @ np.vstack([pts, np.ones((1, 500))]))[:ndim]
+    assert np.abs(out[0] - ref).max() < 1e-12
+
+
+def test_exact_affine_3d():
+    """3D: an affine field with a fixed point at the cube centre, given directly
+    per cell (the oracle takes A; continuity holds since the field is global)."""
+    nc = [2, 2, 2]
+    t = make_tessellation(nc, zero_boundary=True)
+    planes, nbr = t.facet_table()
+    rng = np.random.default_rng(8)
+    M = rng.standard_normal((3, 3)) * 0.25
+    M -= np.eye(3) * 0.6  # contracting: trajectories stay in the cube
+    bvec = -M @ np.full(3, 0.5)
+    A = np.tile(np.concatenate([M, bvec[:, None]], 1), (t.nC, 1, 1))[None]
+    pts = rng.uniform(0.1, 0.9, size=(3, 400))
+    out = O.cf_forward(3, nc, A, planes, nbr, pts)
+    Sq = np.zeros((4, 4))
+    Sq[:3] = np.concatenate([M, bvec[:, None]], 1)
+    ref = (expm(Sq) @ np.vstack([pts, np.ones((1, 400))]))[:3]
+    assert np.abs(out[0] - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("2d_3x3", 0.5), ("2d_2x2_nzb", 0.5), ("3d_2x2x2", 0.3)])
+def test_exact_flow_invariants(name, scale):
+    b, nc, ndim, A, theta = _setup(name, 2, scale, seed=9)
+    pts = RNG.uniform(0, 1, size=(ndim, 300))
+    zero = O.cf_forward(ndim, nc, A * 0, b["planes"], b["nbr"], pts)
+    assert np.array_equal(zero[0], pts)
+    full = O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts)
+    half = O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts, T=0.5)
+    twice = np.stack([O.cf_forward(ndim, nc, A[k:k + 1], b["planes"], b["nbr"], half[k], T=0.5)[0]
+                      for k in range(A.shape[0])])
+    assert np.abs(twice - full).max() < 1e-11
+    back = np.stack([O.cf_forward(ndim, nc, -A[k:k + 1], b["planes"], b["nbr"], full[k])[0]
+                     for k in range(A.shape[0])])
+    assert np.abs(back - pts).max() < 1e-11
+    if b["zero_boundary"]:
+        assert full.min() > -1e-12 and full.max() < 1 + 1e-12
+
+
+@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("2d_3x3", 0.5), ("3d_2x2x2", 0.3)])
+def test_reference_scheme_converges_to_exact(name, scale):
+    b, nc, ndim, A, theta = _setup(name, 1, scale, seed=11)
+    pts = RNG.uniform(0, 1, size=(ndim, 200))
+    exact = O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts)
+    errs = [np.abs(O.step_forward_f64(ndim, nc, n, pts, O.trels(A, n)) - exact).max() for n in (50, 500, 5000)]
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 1e-7
+    assert errs[0] / errs[2] > 300  # second order in 1/nstep
+
+
+@pytest.mark.parametrize("name,scale", [("2d_3x3", 0.5), ("3d_2x2x2", 0.3), ("1d_50", 0.6)])
+def test_cell_sequences_match_brute_force(name, scale):
+    """Cells visited by the exact integrator equal those of a fine RK4
+    integration that looks the cell up at every step (segments shorter than the
+    RK4 step are not resolvable by brute force and are ignored)."""
+    b, nc, ndim, A, theta = _setup(name, 1, scale, seed=13)
+    pts = RNG.uniform(0.02, 0.98, size=(ndim, 40))
+    _, seq, dur, nseg = O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts, max_segs=64)
+    h = 1.0 / 20000
+    Ac = A[0]
+
+    def vel(x):
+        c = O.findcell(ndim, nc, x[:, None])[0]
+        return Ac[c, :, :ndim] @ x + Ac[c, :, ndim], c
+
+    for i in range(pts.shape[1]):
+        x = pts[:, i].copy()
+        cells = []
+        for _ in range(20000):
+            k1, c = vel(x)
+            if not cells or cells[-1][0] != c:
+                cells.append([c, 0])
+            cells[-1][1] += 1
+            k2, _ = vel(x + h / 2 * k1)
+            k3, _ = vel(x + h / 2 * k2)
+            k4, _ = vel(x + h * k3)
+            x = x + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        brute = [c for c, n in cells if n >= 3]
+        exact = [int(seq[0, i, s]) for s in range(nseg[0, i]) if dur[0, i, s] >= 3 * h]
+        assert brute == exact, (i, brute, exact)
+
+
+# --------------------------------------------------------------------------
+# exact gradient
+# --------------------------------------------------------------------------
+def _fd_theta(b, nc, ndim, theta, pts, w, ks, eps=1e-6):
+    out = []
+    for k in ks:
+        e = np.zeros_like(theta)
+        e[:, k] = eps
+        fp = O.cf_forward(ndim, nc, O.expand_A(b["B"], theta + e, ndim), b["planes"], b["nbr"], pts)
+        fm = O.cf_forward(ndim, nc, O.expand_A(b["B"], theta - e, ndim), b["planes"], b["nbr"], pts)
+        out.append(((fp - fm) * w).sum(axis=(1, 2)) / (2 * eps))
+    return np.stack(out, 1)
+
+
+@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("2d_3x3", 0.5), ("2d_2x2_nzb", 0.5), ("3d_2x2x2", 0.3)])
+def test_exact_gradient_matches_finite_differences(name, scale):
+    b, nc, ndim, A, theta = _setup(name, 2, scale, seed=15)
+    pts = RNG.uniform(0, 1, size=(ndim, 60))
+    w = RNG.standard_normal((2, ndim, 60))
+    gA, gp = O.cf_backward(ndim, nc, A, b["planes"], b["nbr"], pts, w)
+    gtheta = gA.reshape(2, -1) @ b["B"]
+    ks = list(range(0, theta.shape[1], max(1, theta.shape[1] // 7)))
+    fd = _fd_theta(b, nc, ndim, theta, pts, w, ks)
+    assert np.abs(gtheta[:, ks] - fd).max() <= 1e-6 * max(1.0, np.abs(fd).max())
+    eps = 1e-6
+    for a in range(ndim):
+        e = np.zeros_like(pts)
+        e[a, 7] = eps
+        fp = O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts + e)
+        fm = O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts - e)
+        fd_p = ((fp - fm) * w).sum(axis=(1, 2)) / (2 * eps)
+        assert np.abs(gp[:, a, 7] - fd_p).max() <= 1e-6 * max(1.0, np.abs(fd_p).max())
+
+
+def test_saltation_terms_needed_for_discontinuous_field():
+    """With A perturbed independently per cell the field jumps across facets;
+    then the exact gradient needs the hit-time / cell-change (saltation) terms,
+    and finite differences confirm them."""
+    b = load_basis("2d_3x3")
+    nc = [3, 3]
+    rng = np.random.default_rng(17)
+    theta = rng.standard_normal((1, b["B"].shape[1])) * 0.5
+    A = O.expand_A(b["B"], theta, 2)
+    A = A + rng.standard_normal(A.shape) * 0.05
+    pts = rng.uniform(0.05, 0.95, size=(2, 40))
+    w = rng.standard_normal((1, 2, 40))
+    gA, _ = O.cf_backward(2, nc, A, b["planes"], b["nbr"], pts, w, saltation=True)
+    gA0, _ = O.cf_backward(2, nc, A, b["planes"], b["nbr"], pts, w, saltation=False)
+    eps = 1e-7
+    fd = np.zeros_like(A)
+    for idx in np.ndindex(*A.shape[1:]):
+        e = np.zeros_like(A)
+        e[(0,) + idx] = eps
+        fp = O.cf_forward(2, nc, A + e, b["planes"], b["nbr"], pts)
+        fm = O.cf_forward(2, nc, A - e, b["planes"], b["nbr"], pts)
+        fd[(0,) + idx] = ((fp - fm) * w).sum() / (2 * eps)
+    scale = np.abs(fd).max()
+    assert np.abs(gA - fd).max() <= 1e-5 * scale
+    assert np.abs(gA0 - fd).max() > 1e-3 * scale
+
+
+# --------------------------------------------------------------------------
+# interpolation
+# --------------------------------------------------------------------------
+def test_interpolation_matches_grid_sample_2d():
+    """Inside the domain, libcpab's bilinear interpolation equals grid_sample
+    (align_corners=True) after swapping axes: data[n, c, x, y] is sampled at
+    grid (x, y) in [0,1]^2."""
+    rng = np.random.default_rng(19)
+    data = rng.uniform(size=(2, 3, 9, 13))
+    outsize = [7, 5]
+    grid = rng.uniform(0, 1, size=(2, 2, 35))
+    ref = O.interpolate(2, data, grid, outsize)  # [N, C, 7, 5]
+    img = torch.tensor(data).permute(0, 1, 3, 2)  # [N, C, H=13(y), W=9(x)]
+    g = torch.tensor(grid).permute(0, 2, 1) * 2 - 1  # [N, nP, 2] (x, y) in [-1, 1]
+    s = torch.nn.functional.grid_sample(img, g[:, None], mode="bilinear", align_corners=True,
+                                        padding_mode="border")[:, :, 0]  # [N, C, nP]
+    mine = ref.transpose(0, 1, 3, 2).reshape(2, 3, -1)  # point order x fastest
+    assert np.abs(mine - s.numpy()).max() < 1e-12
+
+
+def test_interpolation_identity_and_linear():
+    rng = np.random.default_rng(21)
+    data = rng.uniform(size=(1, 2, 6, 5, 4))
+    grid = O.uniform_meshgrid([6, 5, 4])[None]
+    assert np.allclose(O.interpolate(3, data, grid, [6, 5, 4]), data, atol=1e-14)
+    # data linear in the index is reproduced exactly at any inside point
+    xs = np.arange(11.0)
+    lin = (3 * xs + 1)[None, None]
+    pts = rng.uniform(0, 1, size=(1, 1, 50))
+    v = O.interpolate(1, lin, pts, [50])
+    assert np.allclose(v[0, 0], 3 * pts[0, 0] * 10 + 1, atol=1e-12)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_interpolation_backward_matches_finite_differences(ndim):
+    rng = np.random.default_rng(23 + ndim)
+    size = {1: [9], 2: [6, 7], 3: [4, 5, 3]}[ndim]
+    outsize = {1: [11], 2: [3, 4], 3: [2, 3, 2]}[ndim]
+    data = rng.uniform(size=(2, 2) + tuple(size))
+    nP = int(np.prod(outsize))
+    grid = rng.uniform(0.01, 0.99, size=(2, ndim, nP))
+    go = rng.standard_normal((2, 2) + tuple(outsize))
+    gd, gg = O.interpolate_backward(ndim, data, grid, outsize, go)
+    eps = 1e-7
+    for _ in range(6):
+        n, a, p = rng.integers(2), rng.integers(ndim), rng.integers(nP)
+        e = np.zeros_like(grid)
+        e[n, a, p] = eps
+        fd = ((O.interpolate(ndim, data, grid + e, outsize) - O.interpolate(ndim, data, grid - e, outsize)) * go).sum()
+        assert abs(gg[n, a, p] - fd / (2 * eps)) < 1e-5 * max(1, abs(fd / (2 * eps)))
+    # linear in data: the data gradient is exact
+    e = rng.standard_normal(data.shape)
+    lhs = (O.interpolate(ndim, e, grid, outsize) * go).sum()
+    assert abs(lhs - (gd * e).sum()) < 1e-10 * max(1, abs(lhs))
