@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark of the CPA transform_data path on one MI355X.
+
+Workload (the 2D image configuration): Cpab([4, 4], zero_boundary=True),
+256 thetas ~ N(0, 0.3^2 I), transform_data of 256 x 3 x 256 x 256 float32
+U(0,1) images to a 256 x 256 output, MSE loss to a U(0,1) target, backward
+w.r.t. theta and the images.  One step = forward + loss + backward of the
+whole batch; a point is one (theta, output pixel) pair.
+
+Prints one JSON line: the batched path's throughput (value), the plain
+one-launch-per-instance path, the forward / gradient error against the FP64
+oracle, the HBM roofline of the dominant kernel, and the reference's own CPU
+op timed on a bounded sample on this host.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+bwd points/s, 2D 4x4 tess: batched vs plain HIP; max err vs FP64 oracle"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(T, n_theta, C, size, device, seed=0):
+    import torch
+    g = torch.Generator().manual_seed(seed)
+    theta = torch.randn(n_theta, T.get_theta_dim(), generator=g) * 0.3
+    data = torch.rand((n_theta, C) + tuple(size), generator=g)
+    target = torch.rand((n_theta, C) + tuple(size), generator=g)
+    return theta.to(device), data.to(device), target.to(device)
+
+
+def run_steps(T, theta, data, target, size, steps):
+    import torch
+    th = theta.detach().clone().requires_grad_(True)
+    dd = data.detach().clone().requires_grad_(True)
+    for _ in range(steps):
+        th.grad = None
+        dd.grad = None
+        out = T.transform_data(dd, th, size)
+        loss = ((out - target) ** 2).mean()
+        loss.backward()
+    return th, dd
+
+
+def time_steps(T, theta, data, target, size, steps, warmup):
+    import torch
+    run_steps(T, theta, data, target, size, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_steps(T, theta, data, target, size, steps)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def time_bwd_kernel(T, theta, data, target, size, reps):
+    """Average duration of the fused backward (k_transform_data_bwd with its
+    two small companions: theta->A and the per-theta reduction), HIP events on
+    the stream the kernels are launched on."""
+    import torch
+    from libcpab_amd import _native as N
+    from libcpab_amd.pytorch import hip_ops as H
+    dev = H.device_tess(T.params, theta.device)
+    n_theta, C = data.shape[:2]
+    nP = int(size[0] * size[1])
+    lin = H.meshgrid_axes(size, theta.device)
+    out = torch.empty_like(data)
+    saved = torch.empty(int(N.lib().cpab_saved_floats(2, n_theta, nP)), device=theta.device)
+    ws, nb = dev.workspace(n_theta, nP)
+    stream = H._stream(theta.device)
+    dsz, osz = H._int_arr(data.shape[2:]), H._int_arr(size)
+    N.check(N.lib().cpab_transform_data_fwd(dev.ref, H._ptr(theta), n_theta, H._ptr(data), C, dsz, osz, H._ptr(lin),
+                                            1.0, H._ptr(out), H._ptr(saved), H._ptr(ws), nb, stream))
+    gout = (2.0 / out.numel()) * (out - target)
+    gdata = torch.zeros_like(data)
+    gth = torch.empty(n_theta, dev.d, device=theta.device)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    fwd_ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def bwd():
+        N.check(N.lib().cpab_transform_data_bwd(dev.ref, H._ptr(theta), n_theta, H._ptr(data), C, dsz, osz,
+                                                H._ptr(lin), 1.0, H._ptr(gout), H._ptr(saved), H._ptr(gdata),
+                                                H._ptr(gth), H._ptr(ws), nb, stream))
+
+    def fwd():
+        N.check(N.lib().cpab_transform_data_fwd(dev.ref, H._ptr(theta), n_theta, H._ptr(data), C, dsz, osz,
+                                                H._ptr(lin), 1.0, H._ptr(out), H._ptr(saved), H._ptr(ws), nb, stream))
+
+    bwd()
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        bwd()
+    ev[1].record()
+    fwd_ev[0].record()
+    for _ in range(reps):
+        fwd()
+    fwd_ev[1].record()
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps, fwd_ev[0].elapsed_time(fwd_ev[1]) / reps
+
+
+def cpu_baseline(T, theta, data, target, size, npts, nerr, seed=1):
+    """The reference's own CPU op (libcpab/core/cpab_ops.cpp, compiled from its
+    sources into oracle/_ref) on a bounded sample of the workload: theta[0],
+    npts random output pixels; forward (nstepsolver = 50), interpolation
+    forward/backward, the reference's RK2 backward and its reduction
+    (libcpab/pytorch/transformer.py:181-202).  Single-threaded like the
+    reference.  Also measures the GPU path's error against the FP64 oracle on
+    the same sample."""
+    import numpy as np
+    import torch
+    import oracle as O
+    p = T.get_params()
+    rng = np.random.default_rng(seed)
+    mesh = O.uniform_meshgrid(size)  # [2, nP], x fastest
+    th0 = theta[0:1].detach().cpu().double().numpy()
+    res = {}
+    # --- GPU error against the FP64 oracle on nerr pixels (theta[0]) ---
+    sel = np.sort(rng.choice(mesh.shape[1], size=nerr, replace=False))
+    pts = mesh[:, sel]
+    A64 = O.expand_A(p.basis, th0, 2)
+    exact = O.cf_forward(2, p.nc, A64, p.facet_planes, p.facet_nbr, pts)
+    g = torch.tensor(pts, dtype=torch.float32, device=theta.device)
+    th = theta[0:1].detach().clone().requires_grad_(True)
+    gpu = T.transform_grid(g, th)
+    w = torch.tensor(rng.standard_normal((1, 2, nerr)), dtype=torch.float32, device=theta.device)
+    (gpu * w).sum().backward()
+    res["max_err_fwd"] = float(np.abs(gpu.detach().cpu().double().numpy() - exact).max())
+    gA, _ = O.cf_backward(2, p.nc, A64, p.facet_planes, p.facet_nbr, pts, w.cpu().double().numpy())
+    gth = gA.reshape(1, -1) @ p.basis
+    res["max_rel_err_grad"] = float(np.abs(th.grad.cpu().double().numpy() - gth).max() / np.abs(gth).max())
+    # --- reference CPU op on npts pixels ---
+    if O.ref_lib() is None:
+        return res, None
+    sel = np.sort(rng.choice(mesh.shape[1], size=npts, replace=False))
+    pts = mesh[:, sel]
+    B32 = p.basis.astype(np.float32)
+    th32 = theta[0:1].detach().cpu().numpy().astype(np.float32)
+    d0 = data[0:1].detach().cpu().double().numpy()
+    tg = target[0:1].detach().cpu().numpy()
+    pts32 = pts.astype(np.float32)
+    t0 = time.perf_counter()
+    A32 = (B32 @ th32.T).T.reshape(1, p.nC, 2, 3).astype(np.float32)
+    trels = O.trels(A32.astype(np.float64), 50).astype(np.float32)
+    fwd = O.ref_forward(2, p.nc, 50, pts32, trels)
+    # interpolation at the warped sample points (values and d/dgrid)
+    outv = O.interpolate(2, d0, fwd.astype(np.float64), [npts, 1])
+    # target value of point p = ix + W*iy lives at target[0, c, ix, iy]
+    tsel = tg.transpose(0, 1, 3, 2).reshape(1, 3, -1)[:, :, sel]
+    go = 2.0 * (outv - tsel.reshape(outv.shape)) / (data.numel())
+    _, ggrid = O.interpolate_backward(2, d0, fwd.astype(np.float64), [npts, 1], go)
+    Bs = O.bases_Bs(B32, p.nC, 2).astype(np.float32)
+    jac = O.ref_backward(2, p.nc, 50, pts32, A32, Bs)  # [d, 1, 2, npts]
+    _ = (jac * ggrid[None].astype(np.float32)).sum(axis=(2, 3)).T
+    dt = time.perf_counter() - t0
+    return res, {"value": npts / dt, "unit": "points/s", "cores": 1, "kind": "reference",
+                 "sample": "theta[0] x %d random output pixels of the 256x256 image workload: the reference's "
+                           "compiled cpab_forward_op (nstepsolver=50) + interpolation fwd/bwd + cpab_backward_op "
+                           "(RK2, d=%d) + reduction; %.1f s" % (npts, p.d, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--kernel-reps", type=int, default=5)
+    ap.add_argument("--cpu-points", type=int, default=40960)
+    ap.add_argument("--err-points", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this path has no multi-GPU mode (the theta batch would be split "
+              "across GPUs with no communication); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from libcpab_amd import Cpab
+
+    dev = torch.device("cuda:0")
+    n_theta, C, size = 256, 3, [256, 256]
+    T = Cpab([4, 4], backend="pytorch", device="gpu", zero_boundary=True, volume_perservation=False)
+    theta, data, target = make_inputs(T, n_theta, C, size, dev)
+    points = n_theta * size[0] * size[1]
+    log("workload ready: %d thetas x %d points" % (n_theta, size[0] * size[1]))
+
+    sec = time_steps(T, theta, data, target, size, args.steps, args.warmup)
+    value = points / sec
+    log("batched: %.3f ms/step, %.3e points/s" % (sec * 1e3, value))
+
+    plain = None
+    if not args.no_plain:
+        T.set_solver_params(launch="plain")
+        psec = time_steps(T, theta, data, target, size, args.plain_steps, 1)
+        T.set_solver_params(launch="batched")
+        plain = {"value": points / psec, "ms_per_step": psec * 1e3}
+        log("plain: %.3f ms/step, %.3e points/s" % (psec * 1e3, points / psec))
+
+    bwd_ms, fwd_ms = time_bwd_kernel(T, theta, data, target, size, args.kernel_reps)
+    algo_bytes = 12 * C * points  # grad_out read + data read + grad_data write, 4 B each, per point
+    achieved = algo_bytes / (bwd_ms * 1e-3) / 1e9
+    log("k_transform_data_bwd call: %.3f ms; forward call %.3f ms" % (bwd_ms, fwd_ms))
+
+    res, cpu = {}, None
+    if not args.no_cpu_baseline:
+        res, cpu = cpu_baseline(T, theta, data, target, size, args.cpu_points, args.err_points)
+        log("cpu baseline:", cpu, res)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "images2d: Cpab([4,4], zero_boundary=True), 256 thetas ~ N(0, 0.3^2 I), "
+                               "transform_data of 256x3x256x256 U(0,1) images, MSE to a U(0,1) target, "
+                               "fwd+bwd w.r.t. theta and images",
+                   "tess": [4, 4], "n_theta": n_theta, "channels": C, "outsize": size, "points_per_step": points},
+        "plain": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
+        "max_err_fwd": res.get("max_err_fwd"),
+        "max_rel_err_grad": res.get("max_rel_err_grad"),
+        "roofline": {"kernel": "k_transform_data_bwd", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": algo_bytes, "launch_ms": bwd_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -121,6 +121,19 @@ int cpab_findcell(int ndim, const int* nc, const float* points, int nP, int* idx
 int cpab_vectorfield(const cpab_tess_t* t, const float* theta, const float* points, int nP, float* v, void* workspace,
                      size_t ws_bytes, hipStream_t stream);
 
+/* The plain method: the same work as cpab_transform_data_fwd/_bwd issued as
+ * one launch sequence per instance (theta).  It is the baseline the batched
+ * single-launch path is measured against; workspace sized for n_theta = 1
+ * suffices.  No reference counterpart (the reference launches once per call,
+ * libcpab/pytorch/transformer_cuda.cu:18-64). */
+int cpab_transform_data_fwd_plain(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                                  const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                                  float* saved, void* workspace, size_t ws_bytes, hipStream_t stream);
+int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                                  const int* dsize, const int* outsize, const float* lin, float t_end,
+                                  const float* grad_out, const float* saved, float* grad_data, float* grad_theta,
+                                  void* workspace, size_t ws_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

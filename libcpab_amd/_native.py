@@ -42,6 +42,11 @@ _SIGNATURES = {
                                          _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "cpab_findcell": (_c_int, [_c_int, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr]),
     "cpab_vectorfield": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_data_fwd_plain": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr,
+                                               _c_ptr, _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_data_bwd_plain": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr,
+                                               _c_ptr, _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                               _c_size, _c_ptr]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -84,6 +84,7 @@ class Cpab(object):
         # this build's solver knobs
         p.method = "closed_form"
         p.store_jacobian = True
+        p.launch = "batched"
         if p.ndim == 1:
             p.nC = p.nc[0]
             p.params_pr_cell = 2
@@ -131,7 +132,7 @@ class Cpab(object):
         return self.params.basis
 
     def set_solver_params(self, nstepsolver=50, numeric_grad=False, use_slow=False, method=None,
-                          store_jacobian=None):
+                          store_jacobian=None, launch=None):
         """libcpab/cpab.py:140-157.  nstepsolver / numeric_grad / use_slow apply
         to method='numeric' (the reference's fixed-step scheme); the default
         method='closed_form' integrates exactly and ignores them."""
@@ -147,6 +148,9 @@ class Cpab(object):
             self.params.method = method
         if store_jacobian is not None:
             self.params.store_jacobian = bool(store_jacobian)
+        if launch is not None:
+            assert launch in ("batched", "plain"), "launch must be 'batched' or 'plain'"
+            self.params.launch = launch
 
     # -- sampling -----------------------------------------------------------
     def uniform_meshgrid(self, n_points):

@@ -715,7 +715,15 @@ Geo make_geo(const cpab_tess_t* t) {
     return g;
 }
 
-int pick_ppt(int nP) { return nP >= 4 * BLOCK * 16 ? 4 : (nP >= 2 * BLOCK * 16 ? 2 : 1); }
+// points per thread: the largest of 4, 2, 1 that still gives >= 4 blocks per CU
+int pick_ppt(int n_theta, int nP) {
+    const long long want = 4LL * 256;
+    for (int ppt = 4; ppt > 1; ppt >>= 1) {
+        const long long blocks = (long long)n_theta * ((nP + BLOCK * ppt - 1) / (BLOCK * ppt));
+        if (blocks >= want) return ppt;
+    }
+    return 1;
+}
 
 size_t lds_tables(int ndim, int nC) {
     switch (ndim) {
@@ -779,7 +787,7 @@ int cpab_abi_version(void) { return CPAB_ABI_VERSION; }
 size_t cpab_workspace_bytes(const cpab_tess_t* t, int n_theta, int nP) {
     if (check_tess(t) != CPAB_OK || n_theta < 0 || nP < 0) return 0;
     const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
-    const int ppt = pick_ppt(nP);
+    const int ppt = pick_ppt(n_theta, nP);
     size_t tiles = (nP + (size_t)BLOCK * ppt - 1) / ((size_t)BLOCK * ppt);
     if (tiles == 0) tiles = 1;
     return align256((size_t)n_theta * Dt * 4) + align256((size_t)n_theta * tiles * Dt * 4);
@@ -815,7 +823,7 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
-    L.ppt = pick_ppt(nP);
+    L.ppt = pick_ppt(n_theta, nP);
     L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -850,7 +858,7 @@ int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
-    L.ppt = pick_ppt(nP);
+    L.ppt = pick_ppt(n_theta, nP);
     L.tiles = nP > 0 ? (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt) : 1;
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -956,7 +964,7 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
-    L.ppt = pick_ppt(nP);
+    L.ppt = pick_ppt(n_theta, nP);
     L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -996,7 +1004,7 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
-    L.ppt = pick_ppt(nP);
+    L.ppt = pick_ppt(n_theta, nP);
     L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -1041,6 +1049,52 @@ int cpab_vectorfield(const cpab_tess_t* t, const float* theta, const float* poin
     CPAB_DISPATCH_D(t->ndim, LAUNCH_VF)
 #undef LAUNCH_VF
     return status_from_hip();
+}
+
+/* One launch sequence per instance: the plain method the batched launches are
+   measured against.  Same arguments and results as the batched calls. */
+int cpab_transform_data_fwd_plain(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                                  const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                                  float* saved, void* workspace, size_t ws_bytes, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    Img im;
+    Src s = {};
+    int nP;
+    if ((rc = img_from(t->ndim, data, C, dsize, im))) return rc;
+    if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
+    const size_t dstride = (size_t)im.s0 * im.s1 * im.s2;
+    const size_t sv = (size_t)(t->ndim + t->ndim * t->ndim) * nP;
+    for (int i = 0; i < n_theta; ++i) {
+        rc = cpab_transform_data_fwd(t, theta + (size_t)i * t->d, 1, data + (size_t)i * C * dstride, C, dsize, outsize,
+                                     lin, t_end, out + (size_t)i * C * nP, saved ? saved + i * sv : nullptr, workspace,
+                                     ws_bytes, stream);
+        if (rc) return rc;
+    }
+    return CPAB_OK;
+}
+
+int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                                  const int* dsize, const int* outsize, const float* lin, float t_end,
+                                  const float* grad_out, const float* saved, float* grad_data, float* grad_theta,
+                                  void* workspace, size_t ws_bytes, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    Img im;
+    Src s = {};
+    int nP;
+    if ((rc = img_from(t->ndim, data, C, dsize, im))) return rc;
+    if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
+    const size_t dstride = (size_t)im.s0 * im.s1 * im.s2;
+    const size_t sv = (size_t)(t->ndim + t->ndim * t->ndim) * nP;
+    for (int i = 0; i < n_theta; ++i) {
+        rc = cpab_transform_data_bwd(t, theta + (size_t)i * t->d, 1, data + (size_t)i * C * dstride, C, dsize, outsize,
+                                     lin, t_end, grad_out + (size_t)i * C * nP, saved ? saved + i * sv : nullptr,
+                                     grad_data ? grad_data + (size_t)i * C * dstride : nullptr,
+                                     grad_theta + (size_t)i * t->d, workspace, ws_bytes, stream);
+        if (rc) return rc;
+    }
+    return CPAB_OK;
 }
 
 }  // extern "C"

@@ -208,8 +208,10 @@ class TransformData(torch.autograd.Function):
         if any(ctx.needs_input_grad[:2]) and getattr(params, "store_jacobian", True):
             saved = torch.empty(int(N.lib().cpab_saved_floats(dev.ndim, n_theta, nP)), device=theta.device,
                                 dtype=torch.float32)
-        ws, nb = dev.workspace(n_theta, nP)
-        N.check(N.lib().cpab_transform_data_fwd(dev.ref, _ptr(theta), n_theta, _ptr(data), C,
+        plain = getattr(params, "launch", "batched") == "plain"
+        ws, nb = dev.workspace(1 if plain else n_theta, nP)
+        fn = N.lib().cpab_transform_data_fwd_plain if plain else N.lib().cpab_transform_data_fwd
+        N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C,
                                                 _int_arr(data.shape[2:]), _int_arr(outsize), _ptr(lin),
                                                 float(t_end), _ptr(out), _ptr(saved), _ptr(ws), nb,
                                                 _stream(theta.device)))
@@ -230,8 +232,10 @@ class TransformData(torch.autograd.Function):
         nP = int(np.prod(ctx.outsize))
         gdata = torch.zeros_like(data) if ctx.needs_input_grad[0] else None
         gtheta = torch.empty(n_theta, dev.d, device=theta.device, dtype=torch.float32)
-        ws, nb = dev.workspace(n_theta, nP)
-        N.check(N.lib().cpab_transform_data_bwd(dev.ref, _ptr(theta), n_theta, _ptr(data), C,
+        plain = getattr(ctx.params, "launch", "batched") == "plain"
+        ws, nb = dev.workspace(1 if plain else n_theta, nP)
+        fn = N.lib().cpab_transform_data_bwd_plain if plain else N.lib().cpab_transform_data_bwd
+        N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C,
                                                 _int_arr(data.shape[2:]), _int_arr(ctx.outsize), _ptr(lin),
                                                 float(ctx.t_end), _ptr(grad), _ptr(saved), _ptr(gdata), _ptr(gtheta),
                                                 _ptr(ws), nb, _stream(theta.device)))
