@@ -8,17 +8,19 @@
 //
 // In-cell flow.  For v = M q + b the exact flow is
 //     q(s) = q0 + sum_{k>=1} s^k / k! M^(k-1) v0 ,   v0 = M q0 + b,
-// an entire series.  It is evaluated on chunks with |M|_F h <= RHO = 1 and
-// KS = 10 terms, so the truncation (< RHO^10/11! relative to the chunk
-// displacement, 2.5e-8) is below FP32 resolution; no eigenvalue case analysis
+// an entire series.  It is evaluated on chunks with |M|_F h <= RHO = 1/2 and
+// KS = 8 terms, so the truncation (< RHO^8/9! relative to the chunk
+// displacement, 1.1e-8) is below FP32 resolution; no eigenvalue case analysis
 // (real / complex / repeated) and no transcendental calls are needed in 2D/3D.
 // 1D uses the closed form q(s) = q0 + s*phi1(a s)*v0 and the analytic hit time
 // s = (qb-q0)/v0 * log1p(r)/r, r = a (qb-q0)/v0.
 //
-// Boundary hits.  For each facet G(s) = n.q(s) + e is the same series; its
-// first root in the chunk is isolated by sampling G and G' at NSAMP points
-// (a local maximum with G > 0 inside a sample interval counts as a crossing)
-// and refined by Newton steps safeguarded by the bracket.
+// Boundary hits.  For each facet G(s) = n.q(s) + e is the same series.  A
+// facet is skipped when a quadratic bound on G over the chunk stays negative;
+// otherwise its first crossing is bracketed by sampling G and G' at NSAMP
+// points (a local maximum with G > 0 inside a sample interval counts).  Only
+// the facet with the earliest bracket is then solved, by Newton steps
+// safeguarded by the bracket.
 //
 // Gradients.  The field is continuous across facets, so the sensitivity of the
 // final point obeys the variational equation with the saltation matrix
@@ -38,8 +40,9 @@
 namespace cpab {
 
 constexpr int NF = 4;           // facet slots per cell
-constexpr int KS = 10;          // series terms
-constexpr float RHO = 1.0f;     // chunk bound |M|_F * h
+constexpr int KS = 8;           // series terms
+constexpr float RHO = 0.5f;     // chunk bound |M|_F * h
+constexpr float GROW = 1.6488f;   // e^RHO: speed growth bound over a chunk
 constexpr int NSAMP = 4;        // samples per chunk for root isolation
 constexpr int MAX_ITERS = 512;  // chunks + crossings per point (termination bound)
 constexpr float ZERO_HOP = 1e-7f;
@@ -66,6 +69,15 @@ struct PointState {
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+// 1/x to ~1 ulp for the safeguarded iterations (their steps are clamped to a
+// bracket, so the correctly rounded quotient is not needed)
+CPAB_HD __forceinline__ float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
 template <int D>
 CPAB_HD __forceinline__ void load_affine(const CellTable<D>& t, int c, float (&M)[D][D], float (&b)[D]) {
     const float* a = t.A + c * (D * (D + 1));
@@ -107,18 +119,19 @@ CPAB_HD __forceinline__ float horner1(const float (&cf)[KS + 1], float s) {
     return gg;
 }
 
-// Root of G in [lo, hi] with G(lo) <= 0 < G(hi): Newton safeguarded by bisection.
+// Root of G in [lo, hi] with G(lo) <= 0 < G(hi): Newton steps from the
+// regula-falsi point, safeguarded by bisection (fixed count: lanes of a wave
+// solve together).
 CPAB_HD __forceinline__ float solve_bracket(const float (&cf)[KS + 1], float lo, float hi, float glo, float ghi) {
-    float s = lo + (hi - lo) * (glo / (glo - ghi));
-    if (!(s > lo && s < hi)) s = 0.5f * (lo + hi);
-    for (int it = 0; it < 24; ++it) {
+    float s = lo + (hi - lo) * (glo * fast_rcp(glo - ghi));
+    if (!(s >= lo && s <= hi)) s = 0.5f * (lo + hi);
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
         float g, d;
         horner2(cf, s, g, d);
-        if (g == 0.f) return s;
         if (g > 0.f) hi = s; else lo = s;
-        float sn = s - g / d;
-        if (!(sn > lo && sn < hi)) sn = 0.5f * (lo + hi);
-        if (fabsf(sn - s) <= 1e-7f * hi || hi - lo <= 1e-7f * hi) { s = sn; break; }
+        float sn = s - g * fast_rcp(d);
+        if (!(sn >= lo && sn <= hi)) sn = 0.5f * (lo + hi);
         s = sn;
     }
     return s;
@@ -129,12 +142,12 @@ CPAB_HD __forceinline__ float solve_bracket(const float (&cf)[KS + 1], float lo,
 // bracket end with the smaller |G'|.
 CPAB_HD __forceinline__ float crit_point(const float (&cf)[KS + 1], float lo, float hi, float dlo, float dhi) {
     int side = 0;
-    for (int it = 0; it < 10; ++it) {
-        const float m = (lo * dhi - hi * dlo) / (dhi - dlo);
+#pragma unroll 1
+    for (int it = 0; it < 4; ++it) {
+        const float m = (lo * dhi - hi * dlo) * fast_rcp(dhi - dlo);
         if (!(m > lo && m < hi)) break;
         float gm, dm;
         horner2(cf, m, gm, dm);
-        if (dm == 0.f) return m;
         if ((dm > 0.f) == (dlo > 0.f)) {
             lo = m;
             dlo = dm;
@@ -150,8 +163,14 @@ CPAB_HD __forceinline__ float crit_point(const float (&cf)[KS + 1], float lo, fl
     return (fabsf(dlo) < fabsf(dhi)) ? lo : hi;
 }
 
-// First time in [0, h] at which the trajectory leaves through the facet with
-// coefficients cf (G(s) = sum_k cf[k] s^k, G <= 0 inside), or -1.
+// Bracket of the first exit through one facet within a chunk.
+struct Bracket {
+    float lo, hi, glo, ghi;
+    int idx;  // 0: exits at the start; i: inside sample interval i; NSAMP+1: none
+};
+
+// Isolates the first time in [0, h] at which the trajectory leaves through the
+// facet with coefficients cf (G(s) = sum_k cf[k] s^k, G <= 0 inside).
 //  * A start on the facet (G(0) >= -tol0) moving out exits at 0 when
 //    allow_start.
 //  * A start on the facet moving in first dips below 0; a later positive
@@ -159,45 +178,57 @@ CPAB_HD __forceinline__ float crit_point(const float (&cf)[KS + 1], float lo, fl
 //  * Any other start on the facet (start exit disallowed, or tangential) is
 //    ignored until G has gone negative.
 //  * Between two negative samples a local maximum above 0 is a crossing.
-CPAB_HD __forceinline__ float first_exit(const float (&cf)[KS + 1], float h, float tol0, float tol1, bool allow_start) {
+CPAB_HD __forceinline__ void first_bracket(const float (&cf)[KS + 1], float h, float tol0, float tol1, bool allow_start,
+                                           Bracket& br) {
+    br.idx = NSAMP + 1;
     const bool on = cf[0] >= -tol0;
-    if (on && allow_start) {
-        if (cf[1] > tol1) return 0.f;
-        if (fabsf(cf[1]) <= tol1 && cf[2] > 0.f) return 0.f;
+    if (on && allow_start && (cf[1] > tol1 || (fabsf(cf[1]) <= tol1 && cf[2] > 0.f))) {
+        br.idx = 0;
+        br.lo = br.hi = 0.f;
+        return;
     }
     bool skip = on && !(cf[1] < -tol1);
     bool dip = on && !skip;
     float ps = 0.f, pg = cf[0], pd = cf[1];
-#pragma unroll
+#pragma unroll 1
     for (int i = 1; i <= NSAMP; ++i) {
         const float s = h * ((float)i * (1.0f / NSAMP));
         float g, d;
         horner2(cf, s, g, d);
+        // 0: nothing, 1: dip minimum (G comes back above 0), 2: graze maximum
+        int crit = 0;
         if (skip) {
             if (g <= 0.f) skip = false;
         } else if (dip) {
-            if (g <= 0.f) {
-                dip = false;
-            } else if (pd < 0.f && d > 0.f) {
-                const float m = crit_point(cf, ps, s, pd, d);
-                const float gm = horner1(cf, m);
-                if (gm < 0.f) return solve_bracket(cf, m, s, gm, g);
-                skip = true;  // never went inside: treat as outside until G < 0
-                dip = false;
-            }
+            if (g <= 0.f) dip = false;
+            else if (pd < 0.f && d > 0.f) crit = 1;
         } else {
-            if (g > 0.f && pg <= 0.f) return solve_bracket(cf, ps, s, pg, g);
-            if (pd > 0.f && d < 0.f) {
-                const float m = crit_point(cf, ps, s, pd, d);
-                const float gm = horner1(cf, m);
-                if (gm > 0.f) return solve_bracket(cf, ps, m, pg, gm);
+            if (g > 0.f && pg <= 0.f) {
+                br.idx = i; br.lo = ps; br.hi = s; br.glo = pg; br.ghi = g;
+                return;
+            }
+            if (pd > 0.f && d < 0.f) crit = 2;
+        }
+        if (crit) {
+            const float m = crit_point(cf, ps, s, pd, d);
+            const float gm = horner1(cf, m);
+            if (crit == 1 && gm < 0.f) {
+                br.idx = i; br.lo = m; br.hi = s; br.glo = gm; br.ghi = g;
+                return;
+            }
+            if (crit == 2 && gm > 0.f) {
+                br.idx = i; br.lo = ps; br.hi = m; br.glo = pg; br.ghi = gm;
+                return;
+            }
+            if (crit == 1) {  // never went inside: treat as outside until G < 0
+                skip = true;
+                dip = false;
             }
         }
         ps = s;
         pg = g;
         pd = d;
     }
-    return -1.f;
 }
 
 // Series vectors W[k] = M^k v0 / (k+1)!, so q(s) = q0 + sum_k s^(k+1) W[k].
@@ -253,6 +284,25 @@ CPAB_HD __forceinline__ void expm_apply(const float (&M)[D][D], float s, const f
     for (int a = 0; a < D; ++a) y[a] = r[a];
 }
 
+// Series coefficients of G(s) = n.q(s) + e for facet f of cell c.
+template <int D>
+CPAB_HD __forceinline__ void facet_coeffs(const CellTable<D>& tab, int c, int f, const float (&q)[D],
+                                          const float (&W)[KS][D], float (&cf)[KS + 1]) {
+    const float4 P = tab.pl[c * NF + f];
+    const float n[3] = {P.x, P.y, P.z};
+    float g0 = P.w;
+#pragma unroll
+    for (int a = 0; a < D; ++a) g0 = fmaf(n[a], q[a], g0);
+    cf[0] = g0;
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc = fmaf(n[a], W[k][a], acc);
+        cf[k + 1] = acc;
+    }
+}
+
 // What one chunk did; handed to the pass callbacks.
 template <int D>
 struct Chunk {
@@ -282,38 +332,67 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
 #pragma unroll
     for (int a = 0; a < D; ++a) vn2 = fmaf(v0[a], v0[a], vn2);
     const float vn = sqrtf(vn2);
-    const float reach = 1.7183f * vn * h + tol0;
     const float tol1 = 1e-6f * vn;
+    // |G''| <= |M|_F |v0| e^RHO on the chunk (|n| = 1)
+    const float k2h2 = 0.5f * GROW * (vn / tab.inv_n[c]) * h * h;
 
-    float best = -1.f;
-    int bf = -1;
-#pragma unroll
+    // phase 1: bracket the first crossing of every reachable facet
+    int bi = NSAMP + 1, bf = -1, bf2 = -1;
+    Bracket b1, b2;
+    b1.lo = b1.hi = b1.glo = b1.ghi = 0.f;
+    b2 = b1;
+#pragma unroll 1
     for (int f = 0; f < NF; ++f) {
         const int nbf = tab.nb[c * NF + f];
         if (nbf < 0) continue;
         const float4 P = tab.pl[c * NF + f];
         const float n[3] = {P.x, P.y, P.z};
-        float g0 = P.w;
+        float g0 = P.w, g1 = 0.f;
 #pragma unroll
-        for (int a = 0; a < D; ++a) g0 = fmaf(n[a], st.q[a], g0);
-        if (g0 < -reach) continue;  // facet out of reach within this chunk
+        for (int a = 0; a < D; ++a) {
+            g0 = fmaf(n[a], st.q[a], g0);
+            g1 = fmaf(n[a], v0[a], g1);
+        }
+        // G(s) <= g0 + g1 s + k2 s^2/2 on [0, h]: skip when that stays below 0
+        if (g0 < -tol0 && fmaf(g1, h, g0) + k2h2 < 0.f) continue;
         float cf[KS + 1];
         cf[0] = g0;
+        cf[1] = g1;
 #pragma unroll
-        for (int k = 0; k < KS; ++k) {
+        for (int k = 1; k < KS; ++k) {
             float acc = 0.f;
 #pragma unroll
             for (int a = 0; a < D; ++a) acc = fmaf(n[a], ch.W[k][a], acc);
             cf[k + 1] = acc;
         }
         const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
-        const float s = first_exit(cf, h, tol0, tol1, allow_start);
-        if (s >= 0.f && (bf < 0 || s < best)) {
-            best = s;
-            bf = f;
+        Bracket br;
+        first_bracket(cf, h, tol0, tol1, allow_start, br);
+        if (br.idx < bi) {
+            bi = br.idx; bf = f; b1 = br; bf2 = -1;
+        } else if (br.idx == bi && bi <= NSAMP && bi > 0) {
+            bf2 = f; b2 = br;
         }
     }
-    const float step = (bf >= 0) ? best : h;
+    // phase 2: solve the earliest bracket (and a tie in the same interval)
+    float best = h;
+    if (bf >= 0) {
+        best = 0.f;
+        if (bi > 0) {
+            float cf[KS + 1];
+            facet_coeffs<D>(tab, c, bf, st.q, ch.W, cf);
+            best = solve_bracket(cf, b1.lo, b1.hi, b1.glo, b1.ghi);
+            if (bf2 >= 0) {
+                facet_coeffs<D>(tab, c, bf2, st.q, ch.W, cf);
+                const float s2 = solve_bracket(cf, b2.lo, b2.hi, b2.glo, b2.ghi);
+                if (s2 < best) {
+                    best = s2;
+                    bf = bf2;
+                }
+            }
+        }
+    }
+    const float step = best;
 #pragma unroll
     for (int a = 0; a < D; ++a) ch.q0[a] = st.q[a];
     series_eval<D>(ch.q0, ch.W, step, ch.q1);
@@ -538,10 +617,8 @@ CPAB_HD __forceinline__ void saltation_adj(const CellTable<D>& tab, const Chunk<
     for (int a = 0; a < D; ++a) mu[a] = fmaf(-n[a], f, mu[a]);
 }
 
-template <int D, bool WITHJ>
-CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
-                                          float (&J)[D][D]) {
-    PointState<D> st;
+template <int D>
+CPAB_HD __forceinline__ void init_point(PointState<D>& st, const float (&q)[D], int c0, float T) {
 #pragma unroll
     for (int a = 0; a < D; ++a) st.q[a] = q[a];
     st.T = T;
@@ -549,41 +626,62 @@ CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const Cel
     st.came_from = -1;
     st.zero_hops = 0;
     st.iters = 0;
-    if constexpr (WITHJ) {
-#pragma unroll
-        for (int r = 0; r < D; ++r)
-#pragma unroll
-            for (int k = 0; k < D; ++k) J[r][k] = (r == k) ? 1.f : 0.f;
-    }
-    while (st.T > 0.f && st.iters < MAX_ITERS) {
-        if constexpr (D == 1) {
-            Chunk1 ch;
-            advance1(st, tab, nx, ch);
-            if constexpr (WITHJ) {
-                J[0][0] *= expf(ch.a * ch.h);
-                if (ch.facet >= 0) {
-                    const float vn = fmaf(tab.A[2 * ch.next], ch.q1, tab.A[2 * ch.next + 1]);
-                    const float vp = fmaf(ch.a, ch.q1, ch.b);
-                    if (fabsf(vp) > 1e-6f * fabsf(vn) && vp != 0.f) J[0][0] *= vn / vp;
-                }
-            }
-        } else {
-            Chunk<D> ch;
-            advance<D>(st, tab, tol0, ch);
-            if constexpr (WITHJ) {
-#pragma unroll
-                for (int j = 0; j < D; ++j) {
-                    float u[D], y[D];
-#pragma unroll
-                    for (int a = 0; a < D; ++a) u[a] = J[a][j];
-                    expm_apply<D>(ch.M, ch.h, u, y);
-#pragma unroll
-                    for (int a = 0; a < D; ++a) J[a][j] = y[a];
-                }
-                if (ch.facet >= 0) saltation_fwd<D>(tab, ch, J);
+}
+
+template <int D>
+CPAB_HD __forceinline__ bool point_active(const PointState<D>& st) {
+    return st.T > 0.f && st.iters < MAX_ITERS;
+}
+
+// One chunk of pass 1: advance the point and carry J = dq(T)/dq0.
+template <int D, bool WITHJ>
+CPAB_HD __forceinline__ void step_point(PointState<D>& st, const CellTable<D>& tab, int nx, float tol0,
+                                        float (&J)[D][D]) {
+    if constexpr (D == 1) {
+        Chunk1 ch;
+        advance1(st, tab, nx, ch);
+        if constexpr (WITHJ) {
+            J[0][0] *= expf(ch.a * ch.h);
+            if (ch.facet >= 0) {
+                const float vn = fmaf(tab.A[2 * ch.next], ch.q1, tab.A[2 * ch.next + 1]);
+                const float vp = fmaf(ch.a, ch.q1, ch.b);
+                if (fabsf(vp) > 1e-6f * fabsf(vn) && vp != 0.f) J[0][0] *= vn / vp;
             }
         }
+    } else {
+        Chunk<D> ch;
+        advance<D>(st, tab, tol0, ch);
+        if constexpr (WITHJ) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                float u[D], y[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) u[a] = J[a][j];
+                expm_apply<D>(ch.M, ch.h, u, y);
+#pragma unroll
+                for (int a = 0; a < D; ++a) J[a][j] = y[a];
+            }
+            if (ch.facet >= 0) saltation_fwd<D>(tab, ch, J);
+        }
     }
+}
+
+template <int D>
+CPAB_HD __forceinline__ void identity(float (&J)[D][D]) {
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int k = 0; k < D; ++k) J[r][k] = (r == k) ? 1.f : 0.f;
+}
+
+// Integrate one point for time T (pass 1 if WITHJ).
+template <int D, bool WITHJ>
+CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+                                       float (&J)[D][D]) {
+    PointState<D> st;
+    init_point<D>(st, q, c0, T);
+    if constexpr (WITHJ) identity<D>(J);
+    while (point_active<D>(st)) step_point<D, WITHJ>(st, tab, nx, tol0, J);
 #pragma unroll
     for (int a = 0; a < D; ++a) q[a] = st.q[a];
 }

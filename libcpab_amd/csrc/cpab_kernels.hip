@@ -388,13 +388,14 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s
 #pragma unroll
         for (int a = 0; a < D; ++a) y[a] = q[a] / (float)nc[a];
         if constexpr (SAVE) {
+            // indexed in output order t (the order threads run in) for coalesced stores
             float* sv = saved + (size_t)th * (D + D * D) * L.nP;
 #pragma unroll
-            for (int a = 0; a < D; ++a) sv[(size_t)a * L.nP + p] = y[a];
+            for (int a = 0; a < D; ++a) sv[(size_t)a * L.nP + t] = y[a];
 #pragma unroll
             for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
-                for (int k = 0; k < D; ++k) sv[(size_t)(D + r2 * D + k) * L.nP + p] = J[r2][k];
+                for (int k = 0; k < D; ++k) sv[(size_t)(D + r2 * D + k) * L.nP + t] = J[r2][k];
         }
         int i0[3], i1[3];
         float fd[3];
@@ -534,11 +535,11 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
             if constexpr (HAVES) {
                 const float* sv = saved + (size_t)th * (D + D * D) * L.nP;
 #pragma unroll
-                for (int a = 0; a < D; ++a) y[a] = sv[(size_t)a * L.nP + p];
+                for (int a = 0; a < D; ++a) y[a] = sv[(size_t)a * L.nP + t];
 #pragma unroll
                 for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
-                    for (int k = 0; k < D; ++k) J[r2][k] = sv[(size_t)(D + r2 * D + k) * L.nP + p];
+                    for (int k = 0; k < D; ++k) J[r2][k] = sv[(size_t)(D + r2 * D + k) * L.nP + t];
             } else {
                 float q[D];
 #pragma unroll
@@ -715,10 +716,10 @@ Geo make_geo(const cpab_tess_t* t) {
     return g;
 }
 
-// points per thread: the largest of 4, 2, 1 that still gives >= 4 blocks per CU
+// points per thread: the largest of 8, 4, 2, 1 that still gives >= 4 blocks per CU
 int pick_ppt(int n_theta, int nP) {
     const long long want = 4LL * 256;
-    for (int ppt = 4; ppt > 1; ppt >>= 1) {
+    for (int ppt = 8; ppt > 1; ppt >>= 1) {
         const long long blocks = (long long)n_theta * ((nP + BLOCK * ppt - 1) / (BLOCK * ppt));
         if (blocks >= want) return ppt;
     }
@@ -956,15 +957,15 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     if (!theta || !lin || !out || n_theta < 0) return CPAB_ERR_ARG;
     if (n_theta == 0) return CPAB_OK;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(n_theta, nP);
     const size_t lds = lds_tables(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     s.lin = lin;
-    Launch L;
-    L.n_theta = n_theta;
-    L.nP = nP;
-    L.ppt = pick_ppt(n_theta, nP);
     L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -993,6 +994,10 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     if (!theta || !lin || !grad_out || !grad_theta || n_theta < 0) return CPAB_ERR_ARG;
     if (n_theta == 0) return CPAB_OK;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(n_theta, nP);
     const size_t lds = lds_bwd(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
@@ -1001,10 +1006,6 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     im.gdata = grad_data;
     s.lin = lin;
-    Launch L;
-    L.n_theta = n_theta;
-    L.nP = nP;
-    L.ppt = pick_ppt(n_theta, nP);
     L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);

@@ -1,0 +1,116 @@
+"""CPU-side checks of the boundary: the HIP library loads without a GPU and
+exports exactly what include/cpab_hip.h declares, argument errors come back
+as status codes (no GPU work is started for them), and the Cpab facade keeps
+the reference's host-side contract."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT, load_basis
+from libcpab_amd import _native as N
+
+HEADER = os.path.join(ROOT, "include", "cpab_hip.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cpab_[A-Za-z0-9_]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert sorted(N.EXPORTED) == _declared()
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.lib()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert lib.cpab_abi_version() == 1
+    assert lib.cpab_status_string(0) == b"ok"
+    assert b"workspace" in lib.cpab_status_string(3)
+
+
+def _tess(name="2d_3x3"):
+    b = load_basis(name)
+    t = N.CpabTess()
+    nc = [int(v) for v in b["nc"]]
+    t.ndim = len(nc)
+    t.nc = N.int3(nc)
+    t.nC = b["nbr"].shape[0]
+    t.d = b["B"].shape[1]
+    t.B = 1  # never dereferenced by the argument checks below
+    t.planes = 1
+    t.nbr = 1
+    return t
+
+
+def test_argument_errors_are_status_codes():
+    L = N.lib()
+    t = _tess()
+    # bad ndim
+    bad = _tess()
+    bad.ndim = 4
+    assert L.cpab_transform_fwd(ctypes.byref(bad), 1, 1, 1, 10, 0, 1.0, 1, None, 1, 1 << 20, None) == 2
+    # inconsistent cell count
+    bad = _tess()
+    bad.nC = 7
+    assert L.cpab_transform_fwd(ctypes.byref(bad), 1, 1, 1, 10, 0, 1.0, 1, None, 1, 1 << 20, None) == 1
+    # missing output
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 1, 1, 10, 0, 1.0, None, None, 1, 1 << 20, None) == 1
+    # workspace too small
+    need = L.cpab_workspace_bytes(ctypes.byref(t), 4, 1000)
+    assert need > 0
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 4, 1, 1000, 0, 1.0, 1, None, 1, need - 1, None) == 3
+    # empty problems are a no-op
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 0, 1, 1000, 0, 1.0, 1, None, None, 0, None) == 0
+    assert L.cpab_interpolate_fwd(5, 1, 1, 1, None, 1, None, 1, None) == 2
+    dsz = (ctypes.c_int * 3)(4, 4, 1)
+    osz = (ctypes.c_int * 3)(0, 4, 1)
+    assert L.cpab_interpolate_fwd(2, 1, 1, 1, dsz, 1, osz, 1, None) == 1
+
+
+def test_workspace_and_saved_sizes():
+    L = N.lib()
+    t = _tess("2d_4x4")
+    D = 64 * 6
+    ws = L.cpab_workspace_bytes(ctypes.byref(t), 256, 65536)
+    assert ws >= 256 * D * 4
+    assert L.cpab_saved_floats(2, 256, 65536) == 256 * 6 * 65536
+
+
+def test_cpab_facade_contract():
+    from libcpab_amd import Cpab, CpabSequential
+    T = Cpab([3, 3], backend="pytorch", device="gpu")
+    assert T.get_theta_dim() == 34 and T.get_theta_size() == 34
+    assert T.get_basis().shape == (216, 34)
+    assert T.identity(3, epsilon=0.1).shape == (3, 34) if torch.cuda.is_available() else True
+    with pytest.raises(AssertionError):
+        Cpab([3, 3], backend="jax")
+    with pytest.raises(AssertionError):
+        Cpab([0, 3])
+    with pytest.raises(NotImplementedError):
+        Cpab([3, 3], backend="numpy", device="cpu")
+    with pytest.raises(NotImplementedError):
+        Cpab([3, 3], backend="pytorch", device="cpu")
+    # tensors on the wrong device are rejected like in the reference
+    with pytest.raises(AssertionError):
+        T.transform_grid(torch.zeros(2, 10), torch.zeros(1, 34))
+    T.set_solver_params(nstepsolver=20, method="numeric")
+    assert T.params.nstepsolver == 20 and T.params.method == "numeric"
+    with pytest.raises(AssertionError):
+        T.set_solver_params(nstepsolver=0)
+    S = CpabSequential(T, Cpab([4, 4]))
+    assert S.get_theta_dim() == [34, 62]
+
+
+def test_meshgrid_order_matches_reference():
+    from libcpab_amd.pytorch.functions import uniform_meshgrid
+    import oracle as O
+    g = uniform_meshgrid(2, [0, 0], [1, 1], [4, 3]).numpy()
+    assert np.allclose(g, O.uniform_meshgrid([4, 3]))
+    assert np.allclose(g[:, :4], [[0, 1 / 3, 2 / 3, 1], [0, 0, 0, 0]])
