@@ -170,6 +170,95 @@ def cpu_baseline(T, theta, data, target, size, npts, nerr, seed=1):
                            "(RK2, d=%d) + reduction; %.1f s" % (npts, p.d, dt)}
 
 
+def ragged_problems(n, seed=0):
+    """The ragged mix: n instances, half 1D (10-100 cells, 256-8192 points,
+    1 channel), half 2D ([2..8] x [2..8] cells, 64^2-512^2 pixels drawn
+    log-uniformly, 3 channels); output size = data size."""
+    import numpy as np
+    from libcpab_amd import Cpab
+    rng = np.random.default_rng(seed)
+    cache = {}
+    probs = []
+    for _ in range(n):
+        if rng.random() < 0.5:
+            nc = [int(rng.integers(10, 101))]
+            size = [int(round(np.exp(rng.uniform(np.log(256), np.log(8192)))))]
+            C = 1
+        else:
+            nc = [int(rng.integers(2, 9)), int(rng.integers(2, 9))]
+            side = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
+            size = [side, side]
+            C = 3
+        key = tuple(nc)
+        if key not in cache:
+            cache[key] = Cpab(nc, backend="pytorch", device="gpu", zero_boundary=True)
+        probs.append((cache[key], C, size, size))
+    return probs
+
+
+def run_ragged(args):
+    import torch
+    from libcpab_amd.batch import CpabBatch
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    probs = ragged_problems(args.instances)
+    b = CpabBatch(probs, device=dev, save=True)
+    g = torch.Generator().manual_seed(0)
+    target = torch.zeros_like(b.out)
+    for i, (T, C, size, out) in enumerate(probs):
+        b.theta_views[i].copy_(torch.randn(T.get_theta_dim(), generator=g) * 0.3)
+        b.data_views[i].copy_(torch.rand([C] + size, generator=g))
+        off = (b.out_views[i].data_ptr() - b.out.data_ptr()) // 4
+        target[off:off + b.out_views[i].numel()].copy_(torch.rand(b.out_views[i].numel(), generator=g))
+    n_used = float(sum(b.used["out"]))
+    log("ragged batch ready: %d instances, %d points (%.1f s)" % (b.n, b.points, time.perf_counter() - t0))
+    best = b.autotune(reps=1)
+    log("autotune: ppt=%d blocks=%d (%.3f ms)" % (best[1], best[2], best[0] * 1e3))
+    b.capture()
+
+    def step_batched():
+        b.replay_forward()
+        loss = ((b.out - target) ** 2).sum() / n_used
+        b.grad_out.copy_((b.out - target) * (2.0 / n_used))
+        b.replay_backward()
+        return loss
+
+    def step_plain():
+        b.forward_plain()
+        loss = ((b.out - target) ** 2).sum() / n_used
+        b.grad_out.copy_((b.out - target) * (2.0 / n_used))
+        b.backward_plain()
+        return loss
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    sec = timed(step_batched, args.steps, args.warmup)
+    psec = timed(step_plain, max(1, args.plain_steps), 1)
+    value = b.points / sec
+    log("ragged batched %.3f ms/step, plain %.3f ms/step" % (sec * 1e3, psec * 1e3))
+    line = {
+        "metric": "fwd+bwd points/s, ragged 1D/2D mix: batched (persistent, graph-captured) vs plain HIP",
+        "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "ragged: %d instances, 50/50 1D (10-100 cells, 256-8192 points, C=1) / 2D "
+                               "([2..8]^2 cells, 64^2-512^2 log-uniform, C=3), thetas ~ N(0, 0.3^2 I), MSE, "
+                               "fwd+bwd w.r.t. theta and data" % b.n,
+                   "instances": b.n, "points_per_step": b.points, "ppt": b.ppt, "blocks": b.blocks},
+        "plain": {"value": b.points / psec, "ms_per_step": psec * 1e3},
+        "batched_over_plain": psec / sec,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,11 +270,16 @@ def main():
     ap.add_argument("--err-points", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", default="images2d", choices=["images2d", "ragged"])
+    ap.add_argument("--instances", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this path has no multi-GPU mode (the theta batch would be split "
               "across GPUs with no communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ragged":
+        return run_ragged(args)
 
     import torch
     from libcpab_amd import Cpab

@@ -134,6 +134,63 @@ int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int 
                                   const float* grad_out, const float* saved, float* grad_data, float* grad_theta,
                                   void* workspace, size_t ws_bytes, hipStream_t stream);
 
+/* ---- ragged batches --------------------------------------------------------
+ * One transform_data problem per instance; instances may differ in ndim,
+ * tessellation, theta size, channels, data and output sizes.  All pointers
+ * are device pointers except that the descriptor array itself is passed both
+ * as a host copy (validation, launch sizing) and a device copy (read by the
+ * kernels).  cpab_batch_plan fills item0/tiles and the item -> instance map. */
+typedef struct cpab_instance {
+    cpab_tess_t tess;
+    int C;
+    int dsize[3];
+    int outsize[3];
+    int nP;
+    int item0;           /* first work item (tile of BLOCK*ppt points); set by cpab_batch_plan */
+    int tiles;           /* work items of this instance; set by cpab_batch_plan */
+    const float* theta;  /* [d] */
+    const float* data;   /* [C, dsize...] */
+    const float* lin;    /* outsize[0] + outsize[1] + outsize[2] meshgrid axis coordinates */
+    float* out;          /* [C, outsize...] */
+    float* saved;        /* [ndim + ndim^2, nP] or NULL */
+    const float* grad_out;
+    float* grad_data;    /* NULL or zero-filled [C, dsize...] */
+    float* grad_theta;   /* [d] */
+    void* ws;            /* cpab_batch_workspace_bytes(&tess, nP, ppt) bytes */
+    size_t ws_bytes;
+} cpab_instance_t;
+
+/* Per-instance workspace that serves both the batched and the plain calls. */
+size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, int nP, int ppt);
+
+/* Splits every instance into tiles of 256*ppt points: writes inst[i].item0 /
+ * .tiles and item_inst[item] (host array of max_items ints).  Returns the
+ * number of work items, or -CPAB_ERR_* . */
+int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, int max_items);
+
+/* One persistent launch over all work items (blocks = grid size; workers pull
+ * items from an atomic queue at counter[0], reset inside the call by a
+ * kernel, so the sequence can be captured into a graph), preceded by the
+ * per-instance A = B theta kernel.  No reference counterpart (the reference
+ * handles one tessellation and one size per call,
+ * libcpab/pytorch/transformer_cuda.cu:18-119). */
+int cpab_batch_fwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_inst, int n_inst,
+                   const int* dev_item_inst, int n_items, int ppt, int blocks, float t_end, int* counter,
+                   hipStream_t stream);
+
+/* Backward of cpab_batch_fwd: dL/ddata (if grad_data), dL/dtheta of every
+ * instance (per-item partials reduced per instance in a fixed order). */
+int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_inst, int n_inst,
+                   const int* dev_item_inst, int n_items, int ppt, int blocks, float t_end, int* counter,
+                   hipStream_t stream);
+
+/* The same work as one launch sequence per instance (the plain method). */
+int cpab_batch_fwd_plain(const cpab_instance_t* host_inst, int n_inst, float t_end, hipStream_t stream);
+int cpab_batch_bwd_plain(const cpab_instance_t* host_inst, int n_inst, float t_end, hipStream_t stream);
+
+/* p[0:n] = 0 with a kernel (safe inside graph capture). */
+int cpab_fill_zero(float* p, size_t n, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,14 @@ class CpabTess(ctypes.Structure):
                 ("B", _c_ptr), ("planes", _c_ptr), ("nbr", _c_ptr)]
 
 
+class CpabInstance(ctypes.Structure):
+    """Mirror of cpab_instance_t."""
+    _fields_ = [("tess", CpabTess), ("C", _c_int), ("dsize", _c_int * 3), ("outsize", _c_int * 3), ("nP", _c_int),
+                ("item0", _c_int), ("tiles", _c_int), ("theta", _c_ptr), ("data", _c_ptr), ("lin", _c_ptr),
+                ("out", _c_ptr), ("saved", _c_ptr), ("grad_out", _c_ptr), ("grad_data", _c_ptr),
+                ("grad_theta", _c_ptr), ("ws", _c_ptr), ("ws_bytes", _c_size)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "cpab_status_string": (ctypes.c_char_p, [_c_int]),
@@ -47,6 +55,13 @@ _SIGNATURES = {
     "cpab_transform_data_bwd_plain": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr,
                                                _c_ptr, _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                                _c_size, _c_ptr]),
+    "cpab_batch_workspace_bytes": (_c_size, [ctypes.POINTER(CpabTess), _c_int, _c_int]),
+    "cpab_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, _c_ptr, _c_int]),
+    "cpab_batch_fwd": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_float, _c_ptr, _c_ptr]),
+    "cpab_batch_bwd": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_float, _c_ptr, _c_ptr]),
+    "cpab_batch_fwd_plain": (_c_int, [_c_ptr, _c_int, _c_float, _c_ptr]),
+    "cpab_batch_bwd_plain": (_c_int, [_c_ptr, _c_int, _c_float, _c_ptr]),
+    "cpab_fill_zero": (_c_int, [_c_ptr, _c_size, _c_ptr]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

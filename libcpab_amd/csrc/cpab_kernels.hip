@@ -86,15 +86,32 @@ __device__ __forceinline__ int out_to_point(const Src& s, int t, int (&ix)[3]) {
     }
 }
 
+// LDS layout of one theta's tables: A_q [nC*P], 1/|M|_F [nC (padded to 4)],
+// facet planes [nC*NF] float4, neighbours [nC*NF] int
+template <int D>
+__device__ __forceinline__ CellTable<D> table_view(float* smem, int nC) {
+    constexpr int P = D * (D + 1);
+    CellTable<D> t;
+    float* sA = smem;
+    float* sInv = sA + nC * P;
+    float4* sPl = reinterpret_cast<float4*>(sInv + ((nC + 3) & ~3));
+    t.A = sA;
+    t.inv_n = sInv;
+    t.pl = sPl;
+    t.nb = reinterpret_cast<int*>(sPl + (D > 1 ? nC * NF : 0));
+    return t;
+}
+
 // stage theta's q-space affine parameters, their inverse norms and the facet table
 template <int D>
 __device__ __forceinline__ CellTable<D> stage_tables(float* smem, const float* Aq, const Geo& g) {
     constexpr int P = D * (D + 1);
     const int nC = g.nC;
-    float* sA = smem;
-    float* sInv = sA + nC * P;
-    float4* sPl = reinterpret_cast<float4*>(sInv + ((nC + 3) & ~3));
-    int* sNb = reinterpret_cast<int*>(sPl + (D > 1 ? nC * NF : 0));
+    CellTable<D> t = table_view<D>(smem, nC);
+    float* sA = const_cast<float*>(t.A);
+    float* sInv = const_cast<float*>(t.inv_n);
+    float4* sPl = const_cast<float4*>(t.pl);
+    int* sNb = const_cast<int*>(t.nb);
     for (int i = threadIdx.x; i < nC * P; i += BLOCK) sA[i] = Aq[i];
     if constexpr (D > 1) {
         for (int i = threadIdx.x; i < nC * NF; i += BLOCK) {
@@ -115,11 +132,6 @@ __device__ __forceinline__ CellTable<D> stage_tables(float* smem, const float* A
         sInv[c] = s > 0.f ? rsqrtf(s) : INFINITY;
     }
     __syncthreads();
-    CellTable<D> t;
-    t.A = sA;
-    t.inv_n = sInv;
-    t.pl = sPl;
-    t.nb = sNb;
     return t;
 }
 
@@ -358,15 +370,12 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
 
 // meshgrid -> integrate -> interpolate.  saved (nullable): [n_theta, D + D*D, nP]
 // holds the warped point (x-space) and the q-space Jacobian for the backward.
+// One tile (points t in [tile*ppt*BLOCK, (tile+1)*ppt*BLOCK)) of one
+// instance's fused forward; data / out / sv point at the instance's slices.
 template <int D, bool SAVE>
-__global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
-                                                          float* __restrict__ out, float* __restrict__ saved) {
-    extern __shared__ float smem[];
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int th = lb / L.tiles, tile = lb % L.tiles;
-    if (th >= L.n_theta) return;
-    const int nC = g.nC;
-    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * D * (D + 1), g);
+__device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, const Src& s, const Img& im,
+                                               const CellTable<D>& tab, int tile, const float* __restrict__ data,
+                                               float* __restrict__ out, float* __restrict__ sv) {
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
@@ -374,7 +383,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s
         const int t = (tile * L.ppt + r) * BLOCK + threadIdx.x;
         if (t >= L.nP) break;
         int ix[3];
-        const int p = out_to_point<D>(s, t, ix);
+        out_to_point<D>(s, t, ix);
         float x[D], q[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -389,7 +398,6 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s
         for (int a = 0; a < D; ++a) y[a] = q[a] / (float)nc[a];
         if constexpr (SAVE) {
             // indexed in output order t (the order threads run in) for coalesced stores
-            float* sv = saved + (size_t)th * (D + D * D) * L.nP;
 #pragma unroll
             for (int a = 0; a < D; ++a) sv[(size_t)a * L.nP + t] = y[a];
 #pragma unroll
@@ -400,11 +408,22 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s
         int i0[3], i1[3];
         float fd[3];
         corners<D>(y, im, i0, i1, fd);
-        for (int c = 0; c < im.C; ++c) {
-            const float* dp = im.data + ((size_t)th * im.C + c) * dstride;
-            out[((size_t)th * im.C + c) * L.nP + t] = interp_value<D>(dp, im, i0, i1, fd);
-        }
+        for (int c = 0; c < im.C; ++c)
+            out[(size_t)c * L.nP + t] = interp_value<D>(data + (size_t)c * dstride, im, i0, i1, fd);
     }
+}
+
+template <int D, bool SAVE>
+__global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
+                                                          float* __restrict__ out, float* __restrict__ saved) {
+    extern __shared__ float smem[];
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * g.nC * D * (D + 1), g);
+    const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    fused_fwd_tile<D, SAVE>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
+                            out + (size_t)th * im.C * L.nP, SAVE ? saved + (size_t)th * (D + D * D) * L.nP : nullptr);
 }
 
 template <int D>
@@ -493,20 +512,13 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
 // adjoint of k_transform_data.  gout [n_theta, C, outsize] in output order;
 // saved (nullable) from the forward.  im.gdata (nullable, zero-filled by the
 // caller) receives dL/ddata by atomic scatter.
+// One tile of one instance's fused backward; adds dL/dA_q of the tile's points
+// into this wave's LDS accumulator wacc (all lanes of the block take part).
 template <int D, bool HAVES>
-__global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
-                                                              const float* __restrict__ gout,
-                                                              const float* __restrict__ saved, float* __restrict__ part) {
-    extern __shared__ float smem[];
-    constexpr int P = D * (D + 1);
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int th = lb / L.tiles, tile = lb % L.tiles;
-    if (th >= L.n_theta) return;
-    const int nC = g.nC;
-    float* acc = smem + table_floats<D>(nC);
-    for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
-    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
-    float* wacc = acc + (threadIdx.x >> 6) * nC * P;
+__device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, const Src& s, const Img& im,
+                                               const CellTable<D>& tab, int tile, const float* __restrict__ data,
+                                               float* __restrict__ gdata, const float* __restrict__ gout,
+                                               const float* __restrict__ sv, float* wacc) {
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const int sz[3] = {im.s0, im.s1, im.s2};
@@ -523,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
         }
         if (valid) {
             int ix[3];
-            const int p = out_to_point<D>(s, t, ix);
+            out_to_point<D>(s, t, ix);
             float x[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
@@ -533,7 +545,6 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
             c0 = find_cell<D>(x, nc);
             float J[D][D], y[D];
             if constexpr (HAVES) {
-                const float* sv = saved + (size_t)th * (D + D * D) * L.nP;
 #pragma unroll
                 for (int a = 0; a < D; ++a) y[a] = sv[(size_t)a * L.nP + t];
 #pragma unroll
@@ -552,9 +563,9 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
             float fd[3], dfd[3] = {0.f, 0.f, 0.f};
             corners<D>(y, im, i0, i1, fd);
             for (int c = 0; c < im.C; ++c) {
-                const size_t base = ((size_t)th * im.C + c) * dstride;
-                const float gv = gout[((size_t)th * im.C + c) * L.nP + t];
-                interp_backward<D>(im.data + base, im.gdata ? im.gdata + base : nullptr, im, i0, i1, fd, gv, dfd);
+                const size_t base = (size_t)c * dstride;
+                interp_backward<D>(data + base, gdata ? gdata + base : nullptr, im, i0, i1, fd,
+                                   gout[(size_t)c * L.nP + t], dfd);
             }
             float w[D];
 #pragma unroll
@@ -569,6 +580,29 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
         }
         pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc);
     }
+}
+
+// adjoint of k_transform_data.  gout [n_theta, C, outsize] in output order;
+// saved (nullable) from the forward.  im.gdata (nullable, zero-filled by the
+// caller) receives dL/ddata by atomic scatter.
+template <int D, bool HAVES>
+__global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
+                                                              const float* __restrict__ gout,
+                                                              const float* __restrict__ saved, float* __restrict__ part) {
+    extern __shared__ float smem[];
+    constexpr int P = D * (D + 1);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const int nC = g.nC;
+    float* acc = smem + table_floats<D>(nC);
+    for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
+    const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
+                             im.gdata ? im.gdata + (size_t)th * im.C * dstride : nullptr,
+                             gout + (size_t)th * im.C * L.nP, HAVES ? saved + (size_t)th * (D + D * D) * L.nP : nullptr,
+                             acc + (threadIdx.x >> 6) * nC * P);
     write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
 }
 
@@ -679,6 +713,184 @@ __global__ __launch_bounds__(BLOCK) void k_vectorfield(Geo g, const float* __res
 #pragma unroll
         for (int k = 0; k < D; ++k) acc = fmaf(A[r * (D + 1) + k], q[k], acc);
         v[(size_t)r * nP + p] = acc / (float)nc[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ragged batches: persistent workers over (instance, tile) items
+// ---------------------------------------------------------------------------
+__host__ __device__ inline size_t align256_hd(size_t b) { return (b + 255) & ~(size_t)255; }
+
+__device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, float t_end, Launch& L, Geo& g, Src& s,
+                                          Img& im) {
+    const int nd = I->tess.ndim;
+    L.n_theta = 1;
+    L.nP = I->nP;
+    L.tiles = I->tiles;
+    L.ppt = ppt;
+    L.t_end = t_end;
+    g.ndim = nd;
+    g.nc0 = I->tess.nc[0];
+    g.nc1 = nd > 1 ? I->tess.nc[1] : 1;
+    g.nc2 = nd > 2 ? I->tess.nc[2] : 1;
+    g.nC = I->tess.nC;
+    g.pl = reinterpret_cast<const float4*>(I->tess.planes);
+    g.nb = I->tess.nbr;
+    int m = g.nc0 > g.nc1 ? g.nc0 : g.nc1;
+    m = m > g.nc2 ? m : g.nc2;
+    g.tol0 = 2e-6f * (float)m;
+    s.pts = nullptr;
+    s.per_theta = 0;
+    s.lin = I->lin;
+    s.g0 = I->outsize[0];
+    s.g1 = nd > 1 ? I->outsize[1] : 1;
+    s.g2 = nd > 2 ? I->outsize[2] : 1;
+    im.data = I->data;
+    im.gdata = I->grad_data;
+    im.C = I->C;
+    im.s0 = I->dsize[0];
+    im.s1 = nd > 1 ? I->dsize[1] : 1;
+    im.s2 = nd > 2 ? I->dsize[2] : 1;
+}
+
+__device__ __forceinline__ int next_item(int* counter, int* s_item) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(*s_item);
+    __syncthreads();
+    return item;
+}
+
+__global__ void k_counter_reset(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_zero(float* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) p[i] = 0.f;
+}
+
+// A_q = B theta for every instance: grid (ceil(maxDt/BLOCK), n_inst)
+__global__ __launch_bounds__(BLOCK) void k_batch_A(const cpab_instance_t* __restrict__ inst) {
+    extern __shared__ float sth[];
+    const cpab_instance_t* I = inst + blockIdx.y;
+    const int nd = I->tess.ndim, d = I->tess.d;
+    const int P = nd * (nd + 1);
+    const int Dt = I->tess.nC * P;
+    if ((int)(blockIdx.x * BLOCK) >= Dt) return;
+    for (int k = threadIdx.x; k < d; k += BLOCK) sth[k] = I->theta[k];
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= Dt) return;
+    const float* b = I->tess.B + (size_t)i * d;
+    float acc = 0.f;
+    for (int k = 0; k < d; ++k) acc = fmaf(b[k], sth[k], acc);
+    const int rem = i % P, r = rem / (nd + 1), col = rem % (nd + 1);
+    const float sc = (col < nd) ? (float)I->tess.nc[r] / (float)I->tess.nc[col] : (float)I->tess.nc[r];
+    static_cast<float*>(I->ws)[i] = acc * sc;
+}
+
+template <bool SAVE>
+__global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __restrict__ inst,
+                                                     const int* __restrict__ item_inst, int n_items, int ppt,
+                                                     float t_end, int* counter) {
+    extern __shared__ float smem[];
+    __shared__ int s_item;
+    int cur = -1;
+    for (;;) {
+        const int item = next_item(counter, &s_item);
+        if (item >= n_items) break;
+        const int ii = __builtin_amdgcn_readfirstlane(item_inst[item]);
+        const cpab_instance_t* I = inst + ii;
+        Launch L;
+        Geo g;
+        Src s;
+        Img im;
+        item_args(I, ppt, t_end, L, g, s, im);
+        const int tile = item - I->item0;
+        const bool restage = ii != cur;
+        cur = ii;
+        const float* Aq = static_cast<const float*>(I->ws);
+        switch (g.ndim) {
+            case 1: {
+                const CellTable<1> tab = restage ? stage_tables<1>(smem, Aq, g) : table_view<1>(smem, g.nC);
+                fused_fwd_tile<1, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+            } break;
+            case 2: {
+                const CellTable<2> tab = restage ? stage_tables<2>(smem, Aq, g) : table_view<2>(smem, g.nC);
+                fused_fwd_tile<2, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+            } break;
+            default: {
+                const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC);
+                fused_fwd_tile<3, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+            } break;
+        }
+    }
+}
+
+template <int D, bool HAVES>
+__device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, const cpab_instance_t* I, bool restage,
+                                               const Launch& L, const Geo& g, const Src& s, const Img& im, int tile) {
+    constexpr int P = D * (D + 1);
+    const int nCP = g.nC * P;
+    float* acc = acc_base;
+    for (int i = threadIdx.x; i < WAVES * nCP; i += BLOCK) acc[i] = 0.f;
+    const CellTable<D> tab = restage ? stage_tables<D>(smem, static_cast<const float*>(I->ws), g) : table_view<D>(smem, g.nC);
+    if (!restage) __syncthreads();
+    fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data, im.gdata, I->grad_out, I->saved,
+                             acc + (threadIdx.x >> 6) * nCP);
+    float* part = reinterpret_cast<float*>(static_cast<char*>(I->ws) + align256_hd((size_t)nCP * 4));
+    write_partials<D>(acc, part + (size_t)tile * nCP, nCP);
+}
+
+template <bool HAVES>
+__global__ __launch_bounds__(BLOCK) void k_batch_bwd(const cpab_instance_t* __restrict__ inst,
+                                                     const int* __restrict__ item_inst, int n_items, int ppt,
+                                                     float t_end, int* counter, int acc_offset) {
+    extern __shared__ float smem[];
+    __shared__ int s_item;
+    int cur = -1;
+    for (;;) {
+        const int item = next_item(counter, &s_item);
+        if (item >= n_items) break;
+        const int ii = __builtin_amdgcn_readfirstlane(item_inst[item]);
+        const cpab_instance_t* I = inst + ii;
+        Launch L;
+        Geo g;
+        Src s;
+        Img im;
+        item_args(I, ppt, t_end, L, g, s, im);
+        const int tile = item - I->item0;
+        const bool restage = ii != cur;
+        cur = ii;
+        float* acc = smem + acc_offset;  // after the largest table of the batch
+        switch (g.ndim) {
+            case 1: batch_bwd_item<1, HAVES>(smem, acc, I, restage, L, g, s, im, tile); break;
+            case 2: batch_bwd_item<2, HAVES>(smem, acc, I, restage, L, g, s, im, tile); break;
+            default: batch_bwd_item<3, HAVES>(smem, acc, I, restage, L, g, s, im, tile); break;
+        }
+    }
+}
+
+// per instance: dL/dtheta = B^T (sum over its items of the partials, in x-space)
+__global__ __launch_bounds__(BLOCK) void k_batch_reduce(const cpab_instance_t* __restrict__ inst) {
+    extern __shared__ float sg[];
+    const cpab_instance_t* I = inst + blockIdx.x;
+    const int nd = I->tess.ndim, d = I->tess.d;
+    const int P = nd * (nd + 1);
+    const int Dt = I->tess.nC * P;
+    const float* part = reinterpret_cast<const float*>(static_cast<const char*>(I->ws) + align256_hd((size_t)Dt * 4));
+    for (int i = threadIdx.x; i < Dt; i += BLOCK) {
+        float s = 0.f;
+        for (int t = 0; t < I->tiles; ++t) s += part[(size_t)t * Dt + i];
+        const int rem = i % P, r = rem / (nd + 1), col = rem % (nd + 1);
+        const float sc = (col < nd) ? (float)I->tess.nc[r] / (float)I->tess.nc[col] : (float)I->tess.nc[r];
+        sg[i] = s * sc;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < d; k += BLOCK) {
+        float s = 0.f;
+        for (int i = 0; i < Dt; ++i) s = fmaf(I->tess.B[(size_t)i * d + k], sg[i], s);
+        I->grad_theta[k] = s;
     }
 }
 
@@ -1096,6 +1308,142 @@ int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int 
         if (rc) return rc;
     }
     return CPAB_OK;
+}
+
+size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, int nP, int ppt) {
+    if (check_tess(t) != CPAB_OK || nP < 0 || ppt < 1) return 0;
+    const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
+    size_t tiles = (nP + (size_t)BLOCK * ppt - 1) / ((size_t)BLOCK * ppt);
+    if (tiles == 0) tiles = 1;
+    const size_t batched = align256(Dt * 4) + align256(tiles * Dt * 4);
+    const size_t plain = cpab_workspace_bytes(t, 1, nP);
+    return batched > plain ? batched : plain;
+}
+
+int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, int max_items) {
+    if (!inst || n_inst < 0 || ppt < 1 || !item_inst) return -CPAB_ERR_ARG;
+    long long item = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        cpab_instance_t& I = inst[i];
+        int rc = check_tess(&I.tess);
+        if (rc) return -rc;
+        int tiles = (I.nP + BLOCK * ppt - 1) / (BLOCK * ppt);
+        if (tiles < 1) tiles = 1;
+        if (item + tiles > max_items) return -CPAB_ERR_WORKSPACE;
+        I.item0 = (int)item;
+        I.tiles = tiles;
+        for (int k = 0; k < tiles; ++k) item_inst[item + k] = i;
+        item += tiles;
+    }
+    return (int)item;
+}
+
+static int batch_check(const cpab_instance_t* h, int n_inst, int ppt, size_t& lds_tab, size_t& lds_acc, int& maxd,
+                       int& maxDt, bool bwd) {
+    lds_tab = 0;
+    lds_acc = 0;
+    maxd = 1;
+    maxDt = 1;
+    for (int i = 0; i < n_inst; ++i) {
+        const cpab_instance_t& I = h[i];
+        int rc = check_tess(&I.tess);
+        if (rc) return rc;
+        if (!I.theta || !I.data || !I.lin || !I.out || !I.ws) return CPAB_ERR_ARG;
+        if (bwd && (!I.grad_out || !I.grad_theta)) return CPAB_ERR_ARG;
+        if (I.ws_bytes < cpab_batch_workspace_bytes(&I.tess, I.nP, ppt)) return CPAB_ERR_WORKSPACE;
+        const int nd = I.tess.ndim;
+        const size_t Dt = (size_t)I.tess.nC * nd * (nd + 1);
+        const size_t tb = lds_tables(nd, I.tess.nC);
+        lds_tab = tb > lds_tab ? tb : lds_tab;
+        lds_acc = (size_t)WAVES * Dt * 4 > lds_acc ? (size_t)WAVES * Dt * 4 : lds_acc;
+        maxd = I.tess.d > maxd ? I.tess.d : maxd;
+        maxDt = (int)Dt > maxDt ? (int)Dt : maxDt;
+    }
+    lds_tab = align256(lds_tab);
+    if (lds_tab + (bwd ? lds_acc : 0) > LDS_LIMIT) return CPAB_ERR_LDS;
+    return CPAB_OK;
+}
+
+int cpab_batch_fwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_inst, int n_inst,
+                   const int* dev_item_inst, int n_items, int ppt, int blocks, float t_end, int* counter,
+                   hipStream_t stream) {
+    if (!host_inst || !dev_inst || !dev_item_inst || !counter || n_inst < 0 || n_items < 0 || ppt < 1 || blocks < 1)
+        return CPAB_ERR_ARG;
+    if (n_inst == 0) return CPAB_OK;
+    size_t lds_tab, lds_acc;
+    int maxd, maxDt;
+    int rc = batch_check(host_inst, n_inst, ppt, lds_tab, lds_acc, maxd, maxDt, false);
+    if (rc) return rc;
+    bool save = true;
+    for (int i = 0; i < n_inst; ++i) save = save && host_inst[i].saved != nullptr;
+    hipLaunchKernelGGL(k_batch_A, dim3((maxDt + BLOCK - 1) / BLOCK, n_inst), dim3(BLOCK), maxd * sizeof(float), stream,
+                       dev_inst);
+    hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, stream, counter);
+    if (save)
+        hipLaunchKernelGGL(k_batch_fwd<true>, dim3(blocks), dim3(BLOCK), lds_tab, stream, dev_inst, dev_item_inst,
+                           n_items, ppt, t_end, counter);
+    else
+        hipLaunchKernelGGL(k_batch_fwd<false>, dim3(blocks), dim3(BLOCK), lds_tab, stream, dev_inst, dev_item_inst,
+                           n_items, ppt, t_end, counter);
+    return status_from_hip();
+}
+
+int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_inst, int n_inst,
+                   const int* dev_item_inst, int n_items, int ppt, int blocks, float t_end, int* counter,
+                   hipStream_t stream) {
+    if (!host_inst || !dev_inst || !dev_item_inst || !counter || n_inst < 0 || n_items < 0 || ppt < 1 || blocks < 1)
+        return CPAB_ERR_ARG;
+    if (n_inst == 0) return CPAB_OK;
+    size_t lds_tab, lds_acc;
+    int maxd, maxDt;
+    int rc = batch_check(host_inst, n_inst, ppt, lds_tab, lds_acc, maxd, maxDt, true);
+    if (rc) return rc;
+    bool have = true;
+    for (int i = 0; i < n_inst; ++i) have = have && host_inst[i].saved != nullptr;
+    hipLaunchKernelGGL(k_batch_A, dim3((maxDt + BLOCK - 1) / BLOCK, n_inst), dim3(BLOCK), maxd * sizeof(float), stream,
+                       dev_inst);
+    hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, stream, counter);
+    const int acc_off = (int)(lds_tab / 4);
+    if (have)
+        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(blocks), dim3(BLOCK), lds_tab + lds_acc, stream, dev_inst,
+                           dev_item_inst, n_items, ppt, t_end, counter, acc_off);
+    else
+        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(blocks), dim3(BLOCK), lds_tab + lds_acc, stream, dev_inst,
+                           dev_item_inst, n_items, ppt, t_end, counter, acc_off);
+    if ((rc = status_from_hip())) return rc;
+    hipLaunchKernelGGL(k_batch_reduce, dim3(n_inst), dim3(BLOCK), maxDt * sizeof(float), stream, dev_inst);
+    return status_from_hip();
+}
+
+int cpab_batch_fwd_plain(const cpab_instance_t* host_inst, int n_inst, float t_end, hipStream_t stream) {
+    if (!host_inst || n_inst < 0) return CPAB_ERR_ARG;
+    for (int i = 0; i < n_inst; ++i) {
+        const cpab_instance_t& I = host_inst[i];
+        int rc = cpab_transform_data_fwd(&I.tess, I.theta, 1, I.data, I.C, I.dsize, I.outsize, I.lin, t_end, I.out,
+                                         I.saved, I.ws, I.ws_bytes, stream);
+        if (rc) return rc;
+    }
+    return CPAB_OK;
+}
+
+int cpab_batch_bwd_plain(const cpab_instance_t* host_inst, int n_inst, float t_end, hipStream_t stream) {
+    if (!host_inst || n_inst < 0) return CPAB_ERR_ARG;
+    for (int i = 0; i < n_inst; ++i) {
+        const cpab_instance_t& I = host_inst[i];
+        int rc = cpab_transform_data_bwd(&I.tess, I.theta, 1, I.data, I.C, I.dsize, I.outsize, I.lin, t_end,
+                                         I.grad_out, I.saved, I.grad_data, I.grad_theta, I.ws, I.ws_bytes, stream);
+        if (rc) return rc;
+    }
+    return CPAB_OK;
+}
+
+int cpab_fill_zero(float* p, size_t n, hipStream_t stream) {
+    if (!p && n) return CPAB_ERR_ARG;
+    if (n == 0) return CPAB_OK;
+    size_t blocks = (n + BLOCK - 1) / BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(BLOCK), 0, stream, p, n);
+    return status_from_hip();
 }
 
 }  // extern "C"
