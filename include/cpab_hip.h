@@ -191,6 +191,26 @@ int cpab_batch_bwd_plain(const cpab_instance_t* host_inst, int n_inst, float t_e
 /* p[0:n] = 0 with a kernel (safe inside graph capture). */
 int cpab_fill_zero(float* p, size_t n, hipStream_t stream);
 
+/* ---- method='numeric': the reference's fixed-step scheme ------------------ */
+
+/* T = expm(scale * A~)[:ndim] for n affine matrices A [n, ndim, ndim+1]
+ * (Trels = expm(A~/nstepsolver), libcpab/pytorch/transformer.py:152-155,
+ * which computes it with the Pade-13 of libcpab/pytorch/expm.py). */
+int cpab_expm_affine(const float* A, int n, int ndim, float scale, float* T, hipStream_t stream);
+
+/* Same arguments and results as the reference's cpab_forward_op
+ * (libcpab/core/cpab_ops.h:13-16, libcpab/core/cpab_ops.cpp:224-260) on device
+ * pointers; nstepsolver by value, nc a host array.  Bit-exact with the
+ * reference's op on the same Trels. */
+int cpab_ref_forward(float* newpoints, const float* points, const float* trels, int nstepsolver, const int* nc,
+                     int ndim, int nP, int batch_size, int broadcast, hipStream_t stream);
+
+/* Same as the reference's cpab_backward_op (libcpab/core/cpab_ops.h:17-21,
+ * libcpab/core/cpab_ops.cpp:262-372): grad [d, n_theta, ndim, nP] is written
+ * completely (no zero-fill needed). */
+int cpab_ref_backward(float* grad, const float* points, const float* As, const float* Bs, int nstepsolver,
+                      const int* nc, int n_theta, int d, int ndim, int nP, int nC, int broadcast, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

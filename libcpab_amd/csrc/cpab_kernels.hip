@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "cpab_device.hpp"
+#include "cpab_numeric.hpp"
 #include "../../include/cpab_hip.h"
 
 using namespace cpab;
@@ -895,6 +896,163 @@ __global__ __launch_bounds__(BLOCK) void k_batch_reduce(const cpab_instance_t* _
 }
 
 // ---------------------------------------------------------------------------
+// method='numeric': the reference's fixed-step scheme
+// ---------------------------------------------------------------------------
+// expm(scale * A~) for n affine matrices A [D, D+1] (homogeneous last row 0),
+// FP64 scaling and squaring with a degree-12 Taylor polynomial; top D rows out.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_expm_affine(const float* __restrict__ A, int n, float scale,
+                                                       float* __restrict__ T) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    constexpr int E = D + 1;
+    double X[E][E], R[E][E], Pw[E][E], tmp[E][E];
+    double nrm = 0.0;
+#pragma unroll
+    for (int r = 0; r < E; ++r)
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            X[r][c] = (r < D) ? (double)scale * (double)A[(size_t)i * D * E + r * E + c] : 0.0;
+            nrm += fabs(X[r][c]);
+        }
+    int sq = 0;
+    while (nrm > 0.25 && sq < 60) {
+        nrm *= 0.5;
+        ++sq;
+    }
+    const double f = ldexp(1.0, -sq);
+#pragma unroll
+    for (int r = 0; r < E; ++r)
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            X[r][c] *= f;
+            R[r][c] = (r == c) ? 1.0 : 0.0;
+            Pw[r][c] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int k = 1; k <= 12; ++k) {
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < E; ++m) acc += Pw[r][m] * X[m][c];
+                tmp[r][c] = acc / k;
+            }
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                Pw[r][c] = tmp[r][c];
+                R[r][c] += tmp[r][c];
+            }
+    }
+    for (int k = 0; k < sq; ++k) {
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < E; ++m) acc += R[r][m] * R[m][c];
+                tmp[r][c] = acc;
+            }
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+#pragma unroll
+            for (int c = 0; c < E; ++c) R[r][c] = tmp[r][c];
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < E; ++c) T[(size_t)i * D * E + r * E + c] = (float)R[r][c];
+}
+
+// libcpab/core/cpab_ops.cpp:224-260: grid (ceil(nP/BLOCK), batch); Trels of the
+// block's theta staged in LDS
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_ref_forward(float* __restrict__ newpoints, const float* __restrict__ points,
+                                                       const float* __restrict__ trels, int nstep, int nc0, int nc1,
+                                                       int nc2, int nC, int nP, int broadcast) {
+    extern __shared__ float sT[];
+    constexpr int P = D * (D + 1);
+    const int t = blockIdx.y;
+    for (int k = threadIdx.x; k < nC * P; k += BLOCK) sT[k] = trels[(size_t)t * nC * P + k];
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nP) return;
+    const int nc[3] = {nc0, nc1, nc2};
+    float pt[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) pt[a] = points[(size_t)broadcast * t * nP * D + i + (size_t)a * nP];
+    for (int n = 0; n < nstep; ++n) {
+        const int idx = cpab_ref::find<D>(pt, nc);
+        float np_[D];
+        cpab_ref::affine<D>(np_, sT + P * idx, pt);
+#pragma unroll
+        for (int a = 0; a < D; ++a) pt[a] = np_[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) newpoints[(size_t)t * D * nP + i + (size_t)a * nP] = pt[a];
+}
+
+// libcpab/core/cpab_ops.cpp:262-372: one thread per (point, theta, k);
+// grid (ceil(nP/BLOCK), n_theta, d); grad [d, n_theta, D, nP]
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_ref_backward(float* __restrict__ grad, const float* __restrict__ points,
+                                                        const float* __restrict__ As, const float* __restrict__ Bs,
+                                                        int nstep, int nc0, int nc1, int nc2, int n_theta, int nC,
+                                                        int nP, int broadcast) {
+#pragma clang fp contract(off)
+    extern __shared__ float sAB[];
+    constexpr int P = D * (D + 1);
+    const int b = blockIdx.y, k = blockIdx.z;
+    float* sA = sAB;
+    float* sB = sAB + nC * P;
+    for (int j = threadIdx.x; j < nC * P; j += BLOCK) {
+        sA[j] = As[(size_t)b * nC * P + j];
+        sB[j] = Bs[(size_t)k * nC * P + j];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nP) return;
+    const int nc[3] = {nc0, nc1, nc2};
+    float p[D], q[D], v[D], pm[D], vm[D], qm[D], bt[D], aq[D], u[D], um[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        p[a] = points[(size_t)broadcast * b * D * nP + i + (size_t)a * nP];
+        q[a] = 0.f;
+    }
+    const double h = 1.0 / nstep;
+    for (int s = 0; s < nstep; ++s) {
+        const int cell = cpab_ref::find<D>(p, nc);
+        const float* Al = sA + P * cell;
+        const float* Bl = sB + P * cell;
+        cpab_ref::affine<D>(v, Al, p);
+#pragma unroll
+        for (int a = 0; a < D; ++a) pm[a] = (float)(p[a] + h * v[a] / 2.0);
+        cpab_ref::affine<D>(vm, Al, pm);
+        cpab_ref::affine<D>(bt, Bl, p);
+        cpab_ref::linear<D>(aq, Al, q);
+#pragma unroll
+        for (int a = 0; a < D; ++a) u[a] = bt[a] + aq[a];
+#pragma unroll
+        for (int a = 0; a < D; ++a) qm[a] = (float)(q[a] + h * u[a] / 2.0);
+        cpab_ref::affine<D>(bt, Bl, pm);
+        cpab_ref::linear<D>(aq, Al, qm);
+#pragma unroll
+        for (int a = 0; a < D; ++a) um[a] = bt[a] + aq[a];
+#pragma unroll
+        for (int a = 0; a < D; ++a) q[a] = (float)(q[a] + um[a] * h);
+#pragma unroll
+        for (int a = 0; a < D; ++a) p[a] = (float)(p[a] + vm[a] * h);
+    }
+    const size_t box = (size_t)D * nP * n_theta;
+#pragma unroll
+    for (int a = 0; a < D; ++a) grad[(size_t)k * box + (size_t)D * nP * b + i + (size_t)a * nP] = q[a];
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 int cells_of(int ndim, const int* nc) {
@@ -1443,6 +1601,54 @@ int cpab_fill_zero(float* p, size_t n, hipStream_t stream) {
     size_t blocks = (n + BLOCK - 1) / BLOCK;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(BLOCK), 0, stream, p, n);
+    return status_from_hip();
+}
+
+int cpab_expm_affine(const float* A, int n, int ndim, float scale, float* T, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    if (!A || !T || n < 0) return CPAB_ERR_ARG;
+    if (n == 0) return CPAB_OK;
+    dim3 g((n + BLOCK - 1) / BLOCK);
+#define LAUNCH_EX(DD) hipLaunchKernelGGL((k_expm_affine<DD>), g, dim3(BLOCK), 0, stream, A, n, scale, T);
+    CPAB_DISPATCH_D(ndim, LAUNCH_EX)
+#undef LAUNCH_EX
+    return status_from_hip();
+}
+
+int cpab_ref_forward(float* newpoints, const float* points, const float* trels, int nstepsolver, const int* nc,
+                     int ndim, int nP, int batch_size, int broadcast, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    if (!newpoints || !points || !trels || !nc || nstepsolver < 0 || nP < 0 || batch_size < 0) return CPAB_ERR_ARG;
+    if (nP == 0 || batch_size == 0) return CPAB_OK;
+    const int n0 = nc[0], n1 = ndim > 1 ? nc[1] : 1, n2 = ndim > 2 ? nc[2] : 1;
+    const int nC = (ndim == 1 ? 1 : (ndim == 2 ? 4 : 5)) * n0 * n1 * n2;
+    const size_t lds = (size_t)nC * ndim * (ndim + 1) * 4;
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    dim3 g((nP + BLOCK - 1) / BLOCK, batch_size);
+#define LAUNCH_RF(DD) \
+    hipLaunchKernelGGL((k_ref_forward<DD>), g, dim3(BLOCK), lds, stream, newpoints, points, trels, nstepsolver, n0, n1, \
+                       n2, nC, nP, broadcast ? 1 : 0);
+    CPAB_DISPATCH_D(ndim, LAUNCH_RF)
+#undef LAUNCH_RF
+    return status_from_hip();
+}
+
+int cpab_ref_backward(float* grad, const float* points, const float* As, const float* Bs, int nstepsolver,
+                      const int* nc, int n_theta, int d, int ndim, int nP, int nC, int broadcast, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    if (!grad || !points || !As || !Bs || !nc || nstepsolver < 1 || n_theta < 0 || d < 0 || nP < 0) return CPAB_ERR_ARG;
+    if (nP == 0 || n_theta == 0 || d == 0) return CPAB_OK;
+    const int n0 = nc[0], n1 = ndim > 1 ? nc[1] : 1, n2 = ndim > 2 ? nc[2] : 1;
+    if (nC != (ndim == 1 ? 1 : (ndim == 2 ? 4 : 5)) * n0 * n1 * n2) return CPAB_ERR_ARG;
+    const size_t lds = 2 * (size_t)nC * ndim * (ndim + 1) * 4;
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    if (n_theta > 65535 || d > 65535) return CPAB_ERR_UNSUPPORTED;
+    dim3 g((nP + BLOCK - 1) / BLOCK, n_theta, d);
+#define LAUNCH_RB(DD) \
+    hipLaunchKernelGGL((k_ref_backward<DD>), g, dim3(BLOCK), lds, stream, grad, points, As, Bs, nstepsolver, n0, n1, n2, \
+                       n_theta, nC, nP, broadcast ? 1 : 0);
+    CPAB_DISPATCH_D(ndim, LAUNCH_RB)
+#undef LAUNCH_RB
     return status_from_hip();
 }
 
