@@ -170,6 +170,142 @@ def cpu_baseline(T, theta, data, target, size, npts, nerr, seed=1):
                            "(RK2, d=%d) + reduction; %.1f s" % (npts, p.d, dt)}
 
 
+def oracle_error(T, theta, pts, seed=2):
+    """Max |GPU - FP64 oracle| of transform_grid at pts [ndim, n] for theta[0:1],
+    and the max gradient error relative to the gradient's magnitude."""
+    import numpy as np
+    import torch
+    import oracle as O
+    p = T.get_params()
+    nd = p.ndim
+    rng = np.random.default_rng(seed)
+    A64 = O.expand_A(p.basis, theta[0:1].detach().cpu().double().numpy(), nd)
+    exact = O.cf_forward(nd, p.nc, A64, p.facet_planes, p.facet_nbr, pts)
+    th = theta[0:1].detach().clone().requires_grad_(True)
+    gpu = T.transform_grid(torch.tensor(pts, dtype=torch.float32, device=theta.device), th)
+    w = rng.standard_normal((1, nd, pts.shape[1]))
+    (gpu * torch.tensor(w, dtype=torch.float32, device=theta.device)).sum().backward()
+    gA, _ = O.cf_backward(nd, p.nc, A64, p.facet_planes, p.facet_nbr, pts, w)
+    gth = gA.reshape(1, -1) @ p.basis
+    return (float(np.abs(gpu.detach().cpu().double().numpy() - exact).max()),
+            float(np.abs(th.grad.cpu().double().numpy() - gth).max() / np.abs(gth).max()))
+
+
+def config_inputs(name, device, seed=0):
+    """The 1D time-series, 3D volume and 4-stage sequential configurations:
+    (transforms, thetas, data, target, outsize, data needs grad, description)."""
+    import math
+    import torch
+    from libcpab_amd import Cpab
+    g = torch.Generator().manual_seed(seed)
+    if name == "ts1d":
+        T = Cpab([50], backend="pytorch", device="gpu", zero_boundary=True)
+        n, L = 4096, 1024
+        theta = torch.randn(n, T.get_theta_dim(), generator=g) * 0.5
+        x = torch.linspace(0, 1, L)
+        f = torch.rand(n, 3, 1, generator=g) * 10 + 1
+        ph = torch.rand(n, 3, 1, generator=g) * 2 * math.pi
+        amp = torch.rand(n, 3, 1, generator=g)
+        data = (amp * torch.sin(2 * math.pi * f * x + ph)).sum(1, keepdim=True)
+        data = data + 0.05 * torch.randn(n, 1, L, generator=g)
+        target = torch.rand(n, 1, L, generator=g)
+        desc = ("ts1d: Cpab([50], zero_boundary=True), 4096 thetas ~ N(0, 0.5^2 I), 1024-sample signals "
+                "(3 random sinusoids + N(0, 0.05) noise), MSE to a U(0,1) target, fwd+bwd w.r.t. theta")
+        return [T], [theta.to(device)], data.to(device), target.to(device), [L], False, desc
+    if name == "vol3d":
+        T = Cpab([3, 3, 3], backend="pytorch", device="gpu", zero_boundary=True)
+        n, S = 16, 64
+        theta = torch.randn(n, T.get_theta_dim(), generator=g) * 0.2
+        ax = torch.linspace(0, 1, S)
+        zz, yy, xx = torch.meshgrid(ax, ax, ax, indexing="ij")
+        data = torch.zeros(n, 1, S, S, S)
+        for i in range(n):
+            c = torch.rand(8, 3, generator=g)
+            s = torch.rand(8, generator=g) * 0.15 + 0.05
+            a = torch.rand(8, generator=g) * 0.5 + 0.5
+            for k in range(8):
+                r2 = (xx - c[k, 0]) ** 2 + (yy - c[k, 1]) ** 2 + (zz - c[k, 2]) ** 2
+                data[i, 0] += a[k] * torch.exp(-r2 / (2 * s[k] ** 2))
+        target = torch.rand(n, 1, S, S, S, generator=g)
+        desc = ("vol3d: Cpab([3,3,3], zero_boundary=True), 16 thetas ~ N(0, 0.2^2 I), 16x1x64x64x64 volumes "
+                "(8 random Gaussian blobs each), MSE to a U(0,1) target, fwd+bwd w.r.t. theta")
+        return [T], [theta.to(device)], data.to(device), target.to(device), [S, S, S], False, desc
+    if name == "seq2d":
+        Ts = [Cpab([4, 4], backend="pytorch", device="gpu", zero_boundary=True) for _ in range(4)]
+        n = 64
+        thetas = [(torch.randn(n, Ts[0].get_theta_dim(), generator=g) * 0.3).to(device) for _ in range(4)]
+        data = torch.rand(n, 3, 256, 256, generator=g)
+        target = torch.rand(n, 3, 256, 256, generator=g)
+        desc = ("seq2d: CpabSequential of 4 Cpab([4,4], zero_boundary=True), 64 thetas each ~ N(0, 0.3^2 I), "
+                "64x3x256x256 U(0,1) images, MSE to a U(0,1) target, fwd+bwd w.r.t. all thetas with "
+                "trajectory recomputation")
+        return Ts, thetas, data.to(device), target.to(device), [256, 256], False, desc
+    raise ValueError(name)
+
+
+def run_config(args):
+    """One of the secondary configurations: batched (one launch over all
+    thetas) and plain (one launch per theta) fwd+bwd points/s, and the error
+    against the FP64 oracle on a sample of the workload's grid."""
+    import numpy as np
+    import torch
+    import oracle as O
+    from libcpab_amd import CpabSequential
+    dev = torch.device("cuda:0")
+    Ts, thetas, data, target, size, data_grad, desc = config_inputs(args.workload, dev)
+    n_theta = thetas[0].shape[0]
+    points = n_theta * int(np.prod(size))
+    seq = len(Ts) > 1
+    S = CpabSequential(*Ts) if seq else None
+    log("workload ready: %s" % desc)
+
+    def step():
+        ths = [t.detach().clone().requires_grad_(True) for t in thetas]
+        dd = data.detach().clone().requires_grad_(data_grad)
+        out = S.transform_data(dd, ths, size) if seq else Ts[0].transform_data(dd, ths[0], size)
+        loss = ((out - target) ** 2).mean()
+        loss.backward()
+        return ths
+
+    def timed(steps, warmup):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    sec = timed(args.steps, args.warmup)
+    log("batched: %.3f ms/step, %.3e points/s" % (sec * 1e3, points / sec))
+    plain = None
+    if not args.no_plain and not seq:
+        Ts[0].set_solver_params(launch="plain")
+        psec = timed(args.plain_steps, 1)
+        Ts[0].set_solver_params(launch="batched")
+        plain = {"value": points / psec, "ms_per_step": psec * 1e3}
+        log("plain: %.3f ms/step" % (psec * 1e3))
+    mesh = O.uniform_meshgrid(size)
+    rng = np.random.default_rng(1)
+    pts = mesh[:, np.sort(rng.choice(mesh.shape[1], size=min(args.err_points, mesh.shape[1]), replace=False))]
+    errs = [oracle_error(T, th, pts) for T, th in zip(Ts, thetas)]
+    line = {
+        "metric": "fwd+bwd points/s (%s); max err vs FP64 oracle" % args.workload,
+        "value": points / sec, "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "n_theta": n_theta, "outsize": size, "points_per_step": points,
+                   "stages": len(Ts)},
+        "plain": plain,
+        "batched_over_plain": (points / sec / plain["value"]) if plain else None,
+        "max_err_fwd": max(e[0] for e in errs),
+        "max_rel_err_grad": max(e[1] for e in errs),
+        "err_sample": "theta[0] of each stage, %d grid points" % pts.shape[1],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def ragged_problems(n, seed=0):
     """The ragged mix: n instances, half 1D (10-100 cells, 256-8192 points,
     1 channel), half 2D ([2..8] x [2..8] cells, 64^2-512^2 pixels drawn
@@ -270,7 +406,7 @@ def main():
     ap.add_argument("--err-points", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", default="images2d", choices=["images2d", "ragged"])
+    ap.add_argument("--workload", default="images2d", choices=["images2d", "ragged", "ts1d", "vol3d", "seq2d"])
     ap.add_argument("--instances", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -280,6 +416,8 @@ def main():
 
     if args.workload == "ragged":
         return run_ragged(args)
+    if args.workload != "images2d":
+        return run_config(args)
 
     import torch
     from libcpab_amd import Cpab
