@@ -306,6 +306,25 @@ def run_config(args):
     print(json.dumps(line), flush=True)
 
 
+def run_copy(args):
+    """Device-to-device copy of a 1 GiB buffer: the reference kernel of known
+    traffic (1 GiB read + 1 GiB written per launch) that scales the FETCH_SIZE /
+    WRITE_SIZE counters into bytes."""
+    import torch
+    dev = torch.device("cuda:0")
+    a = torch.ones(256 * 1024 * 1024, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "1 GiB device copy", "value": 2 * a.numel() * 4 / sec / 1e9, "unit": "GB/s",
+                      "ms_per_step": sec * 1e3, "steps": args.steps}), flush=True)
+
+
 def ragged_problems(n, seed=0):
     """The ragged mix: n instances, half 1D (10-100 cells, 256-8192 points,
     1 channel), half 2D ([2..8] x [2..8] cells, 64^2-512^2 pixels drawn
@@ -406,7 +425,7 @@ def main():
     ap.add_argument("--err-points", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", default="images2d", choices=["images2d", "ragged", "ts1d", "vol3d", "seq2d"])
+    ap.add_argument("--workload", default="images2d", choices=["images2d", "ragged", "ts1d", "vol3d", "seq2d", "copy"])
     ap.add_argument("--instances", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -416,6 +435,8 @@ def main():
 
     if args.workload == "ragged":
         return run_ragged(args)
+    if args.workload == "copy":
+        return run_copy(args)
     if args.workload != "images2d":
         return run_config(args)
 

@@ -16,11 +16,15 @@
 // s = (qb-q0)/v0 * log1p(r)/r, r = a (qb-q0)/v0.
 //
 // Boundary hits.  For each facet G(s) = n.q(s) + e is the same series.  A
-// facet is skipped when a quadratic bound on G over the chunk stays negative;
-// otherwise its first crossing is bracketed by sampling G and G' at NSAMP
-// points (a local maximum with G > 0 inside a sample interval counts).  Only
-// the facet with the earliest bracket is then solved, by Newton steps
-// safeguarded by the bracket.
+// facet is skipped when a quadratic bound on G over the chunk stays negative
+// (for a start on the facet moving inwards: when G' stays negative).  Where a
+// bound on the series tail certifies the sign of G'' over the chunk, G(h)
+// decides the crossing and [0, h] brackets a unique root.  Starts on a facet,
+// uncertified curvature and interior maxima are bracketed by sampling G and G'
+// at NSAMP points (a local maximum with G > 0 inside a sample interval
+// counts).  The candidate with the earliest regula-falsi estimate is solved
+// by Newton steps safeguarded by its bracket; another candidate is solved
+// only when it is positive at (or its bracket ends before) that root.
 //
 // Gradients.  The field is continuous across facets, so the sensitivity of the
 // final point obeys the variational equation with the saltation matrix
@@ -333,61 +337,157 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
     for (int a = 0; a < D; ++a) vn2 = fmaf(v0[a], v0[a], vn2);
     const float vn = sqrtf(vn2);
     const float tol1 = 1e-6f * vn;
-    // |G''| <= |M|_F |v0| e^RHO on the chunk (|n| = 1)
-    const float k2h2 = 0.5f * GROW * (vn / tab.inv_n[c]) * h * h;
+    // Bounds on the chunk (|n| = 1, |M| <= |M|_F, |M|_F h <= RHO, |cf_k| <=
+    // |M|^(k-1) |v0| / k!):  |G''(s)| <= K = |M|_F |v0| e^RHO and
+    // |G''(s) - 2 cf2 - 6 cf3 s| <= K4 = |M|_F^3 |v0| h^2 e^RHO / 2.
+    const float mh = h / tab.inv_n[c];
+    const float K = GROW * (vn / tab.inv_n[c]);
+    const float K4 = 0.5f * K * mh * mh;
+    const float k2h2 = 0.5f * K * h * h;
 
-    // phase 1: bracket the first crossing of every reachable facet
-    int bi = NSAMP + 1, bf = -1, bf2 = -1;
-    Bracket b1, b2;
-    b1.lo = b1.hi = b1.glo = b1.ghi = 0.f;
-    b2 = b1;
-#pragma unroll 1
+    // phase 1: classify every facet.  A facet whose first crossing lies in a
+    // bracket with a unique root becomes a candidate; starts on a facet, and
+    // facets whose curvature sign is not certified, take the sampling path.
+    int start_f = -1;
+    unsigned cand = 0u, slow = 0u;
+    float clo[NF], chi[NF], cglo[NF], cghi[NF];
+#pragma unroll
     for (int f = 0; f < NF; ++f) {
+        clo[f] = 0.f;
+        chi[f] = h;
+        cglo[f] = 0.f;
+        cghi[f] = 0.f;
         const int nbf = tab.nb[c * NF + f];
         if (nbf < 0) continue;
         const float4 P = tab.pl[c * NF + f];
         const float n[3] = {P.x, P.y, P.z};
-        float g0 = P.w, g1 = 0.f;
+        float g0 = P.w, g1 = 0.f, g2 = 0.f;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             g0 = fmaf(n[a], st.q[a], g0);
             g1 = fmaf(n[a], v0[a], g1);
         }
-        // G(s) <= g0 + g1 s + k2 s^2/2 on [0, h]: skip when that stays below 0
-        if (g0 < -tol0 && fmaf(g1, h, g0) + k2h2 < 0.f) continue;
+        const bool on = g0 >= -tol0;
+        // G(s) <= g0 + g1 s + K s^2/2 on [0, h]: no crossing when that stays below 0
+        if (!on && fmaf(g1, h, g0) + k2h2 < 0.f) continue;
+        // start on the facet moving inwards with G' < 0 on all of [0, h]: no return
+        if (on && g1 < -tol1 && fmaf(K, h, g1) < 0.f) continue;
+#pragma unroll
+        for (int a = 0; a < D; ++a) g2 = fmaf(n[a], ch.W[1][a], g2);
+        if (on) {
+            const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
+            if (allow_start && (g1 > tol1 || (fabsf(g1) <= tol1 && g2 > 0.f))) {
+                if (start_f < 0) start_f = f;
+            } else {
+                slow |= 1u << f;
+            }
+            continue;
+        }
         float cf[KS + 1];
         cf[0] = g0;
         cf[1] = g1;
+        cf[2] = g2;
 #pragma unroll
-        for (int k = 1; k < KS; ++k) {
+        for (int k = 2; k < KS; ++k) {
             float acc = 0.f;
 #pragma unroll
             for (int a = 0; a < D; ++a) acc = fmaf(n[a], ch.W[k][a], acc);
             cf[k + 1] = acc;
         }
-        const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
-        Bracket br;
-        first_bracket(cf, h, tol0, tol1, allow_start, br);
-        if (br.idx < bi) {
-            bi = br.idx; bf = f; b1 = br; bf2 = -1;
-        } else if (br.idx == bi && bi <= NSAMP && bi > 0) {
-            bf2 = f; b2 = br;
+        // G''(s) = 2 cf2 + 6 cf3 s + R(s), |R| <= K4: the sign of G'' is
+        // certified when the linear part keeps one sign with margin K4
+        const float l0 = 2.f * g2, l1 = fmaf(6.f * cf[3], h, l0);
+        if (!((l0 > K4 && l1 > K4) || (l0 < -K4 && l1 < -K4))) {
+            slow |= 1u << f;
+            continue;
+        }
+        // G convex (g2 > 0) or concave (g2 < 0) on [0, h], G(0) < 0: a positive
+        // G(h) means exactly one crossing; otherwise only a concave G rising
+        // and falling inside the chunk can cross (sampling path).
+        float gh, dh;
+        horner2(cf, h, gh, dh);
+        if (gh > 0.f) {
+            cand |= 1u << f;
+            cglo[f] = g0;
+            cghi[f] = gh;
+        } else if (g2 < 0.f && g1 > 0.f && dh < 0.f) {
+            slow |= 1u << f;
         }
     }
-    // phase 2: solve the earliest bracket (and a tie in the same interval)
-    float best = h;
-    if (bf >= 0) {
-        best = 0.f;
-        if (bi > 0) {
+    // sampling path (starts on a facet, uncertified curvature, interior maxima)
+    if (start_f < 0) {
+#pragma unroll 1
+        while (slow) {
+            const int f = __builtin_ctz(slow);
+            slow &= slow - 1u;
             float cf[KS + 1];
-            facet_coeffs<D>(tab, c, bf, st.q, ch.W, cf);
-            best = solve_bracket(cf, b1.lo, b1.hi, b1.glo, b1.ghi);
-            if (bf2 >= 0) {
-                facet_coeffs<D>(tab, c, bf2, st.q, ch.W, cf);
-                const float s2 = solve_bracket(cf, b2.lo, b2.hi, b2.glo, b2.ghi);
+            facet_coeffs<D>(tab, c, f, st.q, ch.W, cf);
+            const int nbf = tab.nb[c * NF + f];
+            const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
+            Bracket br;
+            first_bracket(cf, h, tol0, tol1, allow_start, br);
+            if (br.idx >= 1 && br.idx <= NSAMP) {
+                cand |= 1u << f;
+#pragma unroll
+                for (int k = 0; k < NF; ++k)
+                    if (k == f) {
+                        clo[k] = br.lo;
+                        chi[k] = br.hi;
+                        cglo[k] = br.glo;
+                        cghi[k] = br.ghi;
+                    }
+            }
+        }
+    }
+    // phase 2: solve the candidate with the earliest regula-falsi estimate, then
+    // any other candidate that is still positive at (or ends before) that root
+    float best = h;
+    int bf = -1;
+    if (start_f >= 0) {
+        best = 0.f;
+        bf = start_f;
+    } else if (cand) {
+        float est = INFINITY, lo = 0.f, hi = 0.f, glo = 0.f, ghi = 0.f;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            if (!((cand >> f) & 1u)) continue;
+            const float e = fmaf(chi[f] - clo[f], cglo[f] * fast_rcp(cglo[f] - cghi[f]), clo[f]);
+            if (e < est) {
+                est = e;
+                bf = f;
+                lo = clo[f];
+                hi = chi[f];
+                glo = cglo[f];
+                ghi = cghi[f];
+            }
+        }
+        float cf[KS + 1];
+        facet_coeffs<D>(tab, c, bf, st.q, ch.W, cf);
+        best = solve_bracket(cf, lo, hi, glo, ghi);
+        unsigned others = cand & ~(1u << bf);
+#pragma unroll 1
+        while (others) {
+            const int f = __builtin_ctz(others);
+            others &= others - 1u;
+#pragma unroll
+            for (int k = 0; k < NF; ++k)
+                if (k == f) {
+                    lo = clo[k];
+                    hi = chi[k];
+                    glo = cglo[k];
+                    ghi = cghi[k];
+                }
+            if (!(lo < best)) continue;
+            facet_coeffs<D>(tab, c, f, st.q, ch.W, cf);
+            if (best < hi) {
+                hi = best;
+                ghi = horner1(cf, best);
+            }
+            if (ghi > 0.f) {
+                const float s2 = solve_bracket(cf, lo, hi, glo, ghi);
                 if (s2 < best) {
                     best = s2;
-                    bf = bf2;
+                    bf = f;
                 }
             }
         }
@@ -754,30 +854,45 @@ CPAB_HD __forceinline__ void chunk_grad1(const Chunk1& ch, float& mu, float (&G)
 // ---------------------------------------------------------------------------
 // wave-level deterministic accumulation of per-lane cell contributions
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// One DPP step: every lane receives v of the lane CTRL selects (0 where there
+// is none or where ROWS masks the row out).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_take(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
+}
+
+// Sum over the 64 lanes in a fixed order: inclusive scans inside each row of
+// 16 (row_shr 1, 2, 4, 8), then the row totals carried by row_bcast:15 and
+// row_bcast:31; lane 63 ends with the total, read into a scalar.
+__device__ __forceinline__ float wave_total(float v) {
+    v += dpp_take<0x111, 0xF>(v);
+    v += dpp_take<0x112, 0xF>(v);
+    v += dpp_take<0x114, 0xF>(v);
+    v += dpp_take<0x118, 0xF>(v);
+    v += dpp_take<0x142, 0xA>(v);
+    v += dpp_take<0x143, 0xC>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Adds g (P values) of every lane with cell >= 0 into acc[cell*P + p] of this
-// wave's LDS region; lanes sharing a cell are summed by a butterfly first, the
-// leader of each group writes.  Must be reached by all lanes of the wave.
+// wave's LDS region; the lanes sharing a cell are summed across the wave in a
+// fixed order, one lane writes.  Must be reached by all lanes of the wave.
 template <int P>
 __device__ __forceinline__ void wave_accumulate(float* acc, int cell, const float (&g)[P]) {
     const int lane = __lane_id();
     uint64_t pending = __ballot(cell >= 0);
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int c0 = __shfl(cell, leader, 64);
+        const int c0 = __builtin_amdgcn_readlane(cell, leader);
         const bool mine = (cell == c0);
-        const uint64_t grp = __ballot(mine);
+        pending &= ~__ballot(mine);
+        float tot[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            float v = wave_sum(mine ? g[p] : 0.f);
-            if (lane == leader) acc[c0 * P + p] += v;
+        for (int p = 0; p < P; ++p) tot[p] = wave_total(mine ? g[p] : 0.f);
+        if (lane == leader) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[c0 * P + p] += tot[p];
         }
-        pending &= ~grp;
     }
 }
 
