@@ -76,7 +76,7 @@ def time_bwd_kernel(T, theta, data, target, size, reps):
     lin = H.meshgrid_axes(size, theta.device)
     out = torch.empty_like(data)
     saved = torch.empty(int(N.lib().cpab_saved_floats(2, n_theta, nP)), device=theta.device)
-    ws, nb = dev.workspace(n_theta, nP)
+    ws, nb = dev.workspace(n_theta, nP, size)
     stream = H._stream(theta.device)
     dsz, osz = H._int_arr(data.shape[2:]), H._int_arr(size)
     N.check(N.lib().cpab_transform_data_fwd(dev.ref, H._ptr(theta), n_theta, H._ptr(data), C, dsz, osz, H._ptr(lin),

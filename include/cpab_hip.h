@@ -3,7 +3,8 @@
  *
  * Every pointer argument except cpab_tess_t itself and the small host arrays
  * dsize / outsize / nc is a device pointer.  The library never allocates:
- * outputs, gradients and the workspace (size from cpab_workspace_bytes) are
+ * outputs, gradients and the workspace (size from cpab_workspace_bytes,
+ * cpab_data_workspace_bytes or cpab_batch_workspace_bytes) are
  * owned by the caller, and nothing is zero-filled inside the library except
  * where stated (grad_data must arrive zero-filled).  Every launch goes to the
  * given stream; no call synchronises the host.  Return value: CPAB_OK (0) or a
@@ -26,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CPAB_ABI_VERSION 1
+#define CPAB_ABI_VERSION 2
 
 #define CPAB_OK 0
 #define CPAB_ERR_ARG 1
@@ -54,8 +55,13 @@ typedef struct cpab_tess {
 const char* cpab_status_string(int status);
 int cpab_abi_version(void);
 
-/* Workspace for one transform/transform_data call on n_theta x nP points. */
+/* Workspace for one transform call (cpab_transform_*, cpab_vectorfield) on
+ * n_theta x nP points. */
 size_t cpab_workspace_bytes(const cpab_tess_t* t, int n_theta, int nP);
+
+/* Workspace for one transform_data call (fused forward / backward, also the
+ * _plain variants with n_theta = 1) onto a meshgrid of outsize[ndim]. */
+size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* outsize);
 
 /* Floats in the forward's saved state for the fused backward: n_theta*(ndim+ndim^2)*nP. */
 size_t cpab_saved_floats(int ndim, int n_theta, int nP);
@@ -123,9 +129,9 @@ int cpab_vectorfield(const cpab_tess_t* t, const float* theta, const float* poin
 
 /* The plain method: the same work as cpab_transform_data_fwd/_bwd issued as
  * one launch sequence per instance (theta).  It is the baseline the batched
- * single-launch path is measured against; workspace sized for n_theta = 1
- * suffices.  No reference counterpart (the reference launches once per call,
- * libcpab/pytorch/transformer_cuda.cu:18-64). */
+ * single-launch path is measured against; cpab_data_workspace_bytes(t, 1,
+ * outsize) bytes of workspace suffice.  No reference counterpart (the
+ * reference launches once per call, libcpab/pytorch/transformer_cuda.cu:18-64). */
 int cpab_transform_data_fwd_plain(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
                                   const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
                                   float* saved, void* workspace, size_t ws_bytes, hipStream_t stream);
@@ -156,16 +162,18 @@ typedef struct cpab_instance {
     const float* grad_out;
     float* grad_data;    /* NULL or zero-filled [C, dsize...] */
     float* grad_theta;   /* [d] */
-    void* ws;            /* cpab_batch_workspace_bytes(&tess, nP, ppt) bytes */
+    void* ws;            /* cpab_batch_workspace_bytes(&tess, outsize, ppt) bytes */
     size_t ws_bytes;
 } cpab_instance_t;
 
 /* Per-instance workspace that serves both the batched and the plain calls. */
-size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, int nP, int ppt);
+size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, const int* outsize, int ppt);
 
-/* Splits every instance into tiles of 256*ppt points: writes inst[i].item0 /
+/* Splits every instance into tiles of 256*ppt points (compact 2D tiles for 2D
+ * grids whose second axis has at least 8 points): writes inst[i].item0 /
  * .tiles and item_inst[item] (host array of max_items ints).  Returns the
- * number of work items, or -CPAB_ERR_* . */
+ * number of work items, or -CPAB_ERR_* .  With item_inst == NULL it only
+ * counts the items. */
 int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, int max_items);
 
 /* One persistent launch over all work items (blocks = grid size; workers pull

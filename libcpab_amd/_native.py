@@ -35,6 +35,7 @@ _SIGNATURES = {
     "cpab_status_string": (ctypes.c_char_p, [_c_int]),
     "cpab_abi_version": (_c_int, []),
     "cpab_workspace_bytes": (_c_size, [ctypes.POINTER(CpabTess), _c_int, _c_int]),
+    "cpab_data_workspace_bytes": (_c_size, [ctypes.POINTER(CpabTess), _c_int, _c_ptr]),
     "cpab_saved_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "cpab_theta_to_A": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_ptr]),
     "cpab_transform_fwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
@@ -55,7 +56,7 @@ _SIGNATURES = {
     "cpab_transform_data_bwd_plain": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr,
                                                _c_ptr, _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                                _c_size, _c_ptr]),
-    "cpab_batch_workspace_bytes": (_c_size, [ctypes.POINTER(CpabTess), _c_int, _c_int]),
+    "cpab_batch_workspace_bytes": (_c_size, [ctypes.POINTER(CpabTess), _c_ptr, _c_int]),
     "cpab_batch_plan": (_c_int, [_c_ptr, _c_int, _c_int, _c_ptr, _c_int]),
     "cpab_batch_fwd": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_float, _c_ptr, _c_ptr]),
     "cpab_batch_bwd": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_float, _c_ptr, _c_ptr]),

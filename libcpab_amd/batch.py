@@ -54,7 +54,7 @@ class CpabBatch:
         n_out = [c * p for c, p in zip(C, nP)]
         n_sv = [(k + k * k) * p for k, p in zip(nd, nP)]
         n_lin = [sum(o) for o in osz]
-        ws_b = [int(L.cpab_batch_workspace_bytes(t.ref, p, 1)) for t, p in zip(self._tess, nP)]
+        ws_b = [int(L.cpab_batch_workspace_bytes(t.ref, H._int_arr(o), 1)) for t, o in zip(self._tess, osz)]
 
         def offsets(sizes, align=64):
             off, o = [], 0
@@ -127,7 +127,9 @@ class CpabBatch:
     # -- launch configuration ------------------------------------------------
     def set_launch(self, ppt, blocks):
         L = N.lib()
-        total = sum((p + 256 * ppt - 1) // (256 * ppt) for p in (self._inst[i].nP for i in range(self.n))) + self.n
+        total = L.cpab_batch_plan(ctypes.cast(self._inst, ctypes.c_void_p), self.n, ppt, None, 0)
+        if total < 0:
+            N.check(-total)
         items = (ctypes.c_int * max(total, 1))()
         n_items = L.cpab_batch_plan(ctypes.cast(self._inst, ctypes.c_void_p), self.n, ppt,
                                     ctypes.cast(items, ctypes.c_void_p), total)

@@ -40,6 +40,7 @@
 
 #define CPAB_HD __host__ __device__
 #include <stdint.h>
+#include <limits.h>
 
 namespace cpab {
 
@@ -872,6 +873,31 @@ __device__ __forceinline__ float wave_total(float v) {
     v += dpp_take<0x142, 0xA>(v);
     v += dpp_take<0x143, 0xC>(v);
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Minimum / maximum over the 64 lanes (same DPP pattern; lanes without a
+// source keep the identity), read into a scalar.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_keep(int ident, int v) {
+    return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ int wave_min(int v) {
+    v = min(v, dpp_keep<0x111, 0xF>(INT_MAX, v));
+    v = min(v, dpp_keep<0x112, 0xF>(INT_MAX, v));
+    v = min(v, dpp_keep<0x114, 0xF>(INT_MAX, v));
+    v = min(v, dpp_keep<0x118, 0xF>(INT_MAX, v));
+    v = min(v, dpp_keep<0x142, 0xA>(INT_MAX, v));
+    v = min(v, dpp_keep<0x143, 0xC>(INT_MAX, v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max(int v) {
+    v = max(v, dpp_keep<0x111, 0xF>(INT_MIN, v));
+    v = max(v, dpp_keep<0x112, 0xF>(INT_MIN, v));
+    v = max(v, dpp_keep<0x114, 0xF>(INT_MIN, v));
+    v = max(v, dpp_keep<0x118, 0xF>(INT_MIN, v));
+    v = max(v, dpp_keep<0x142, 0xA>(INT_MIN, v));
+    v = max(v, dpp_keep<0x143, 0xC>(INT_MIN, v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Adds g (P values) of every lane with cell >= 0 into acc[cell*P + p] of this

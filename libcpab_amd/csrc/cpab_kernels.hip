@@ -52,8 +52,40 @@ struct Img {  // data for interpolation, NC[W(,H(,D))] layout
 
 struct Launch {
     int n_theta, nP, tiles, ppt;
+    int ty;  // 2D output tiles TX x ty (TX = ppt * BLOCK / ty); 0: runs of ppt*BLOCK consecutive points
     float t_end;
 };
+
+// Tiling of a meshgrid output.  2D grids with a second axis of at least 8 go
+// in compact tiles (their warped footprint in the data is compact too, which
+// the backward accumulates in LDS); everything else in consecutive runs.
+__host__ __device__ inline int tile_ty(int ndim, int g1) {
+    if (ndim != 2) return 0;
+    return g1 >= 32 ? 32 : (g1 >= 16 ? 16 : (g1 >= 8 ? 8 : 0));
+}
+
+__host__ __device__ inline int tile_count(int ndim, int nP, int g0, int g1, int ppt) {
+    const int ty = tile_ty(ndim, g1);
+    if (ty == 0) {
+        const int n = (nP + BLOCK * ppt - 1) / (BLOCK * ppt);
+        return n > 0 ? n : 1;
+    }
+    const int tx = ppt * (BLOCK / ty);
+    return ((g0 + tx - 1) / tx) * ((g1 + ty - 1) / ty);
+}
+
+// output-order index t of round r of this thread in tile `tile`, -1 if outside
+__device__ __forceinline__ int tile_t(const Launch& L, const Src& s, int tile, int r) {
+    if (L.ty == 0) {
+        const int t = (tile * L.ppt + r) * BLOCK + (int)threadIdx.x;
+        return t < L.nP ? t : -1;
+    }
+    const int rows = BLOCK / L.ty;
+    const int tiles_y = (s.g1 + L.ty - 1) / L.ty;
+    const int ix = (tile / tiles_y) * (L.ppt * rows) + r * rows + (int)threadIdx.x / L.ty;
+    const int iy = (tile % tiles_y) * L.ty + (int)threadIdx.x % L.ty;
+    return (ix < s.g0 && iy < s.g1) ? ix * s.g1 + iy : -1;
+}
 
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
     // blocks are dispatched round-robin over 8 XCDs; give each XCD a contiguous
@@ -234,6 +266,18 @@ __device__ __forceinline__ float interp_value(const float* __restrict__ dp, cons
     }
 }
 
+// interp_backward's dL/ddata of one channel (2D) into an LDS box whose row r,
+// column k holds data cell (bx0 + r, by0 + k)
+__device__ __forceinline__ void box_add2(float* b, int bh, int bx0, int by0, const int (&i0)[3], const int (&i1)[3],
+                                         const float (&fd)[3], float g) {
+    const float wx0 = 1.f - fd[0], wx1 = fd[0], wy0 = 1.f - fd[1], wy1 = fd[1];
+    const int r0 = (i0[0] - bx0) * bh, r1 = (i1[0] - bx0) * bh, k0 = i0[1] - by0, k1 = i1[1] - by0;
+    atomicAdd(b + r0 + k0, g * wx0 * wy0);
+    atomicAdd(b + r1 + k0, g * wx1 * wy0);
+    atomicAdd(b + r0 + k1, g * wx0 * wy1);
+    atomicAdd(b + r1 + k1, g * wx1 * wy1);
+}
+
 template <int D>
 __device__ __forceinline__ void corners(const float (&y)[D], const Img& im, int (&i0)[3], int (&i1)[3], float (&fd)[3]) {
     const int sz[3] = {im.s0, im.s1, im.s2};
@@ -371,8 +415,8 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
 
 // meshgrid -> integrate -> interpolate.  saved (nullable): [n_theta, D + D*D, nP]
 // holds the warped point (x-space) and the q-space Jacobian for the backward.
-// One tile (points t in [tile*ppt*BLOCK, (tile+1)*ppt*BLOCK)) of one
-// instance's fused forward; data / out / sv point at the instance's slices.
+// One tile (tile_t) of one instance's fused forward; data / out / sv point at
+// the instance's slices.
 template <int D, bool SAVE>
 __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, const Src& s, const Img& im,
                                                const CellTable<D>& tab, int tile, const float* __restrict__ data,
@@ -381,8 +425,8 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
     for (int r = 0; r < L.ppt; ++r) {
-        const int t = (tile * L.ppt + r) * BLOCK + threadIdx.x;
-        if (t >= L.nP) break;
+        const int t = tile_t(L, s, tile, r);
+        if (t < 0) continue;
         int ix[3];
         out_to_point<D>(s, t, ix);
         float x[D], q[D];
@@ -515,18 +559,72 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
 // caller) receives dL/ddata by atomic scatter.
 // One tile of one instance's fused backward; adds dL/dA_q of the tile's points
 // into this wave's LDS accumulator wacc (all lanes of the block take part).
+//
+// dL/ddata of a 2D tile with saved state goes through an LDS box when box
+// (gbox_cap floats, nullable) holds the bounding box of the tile's bilinear
+// footprint for all channels: lanes add into LDS, and the box is flushed with
+// one global atomic per touched cell instead of four per point and channel.
 template <int D, bool HAVES>
 __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, const Src& s, const Img& im,
                                                const CellTable<D>& tab, int tile, const float* __restrict__ data,
                                                float* __restrict__ gdata, const float* __restrict__ gout,
-                                               const float* __restrict__ sv, float* wacc) {
+                                               const float* __restrict__ sv, float* wacc, float* box, int box_cap) {
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const int sz[3] = {im.s0, im.s1, im.s2};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    // footprint box [bb0, bb1] x [bb2, bb3] of the tile (2D, saved state only)
+    __shared__ int s_bb[4];
+    bool use_box = false;
+    int bx0 = 0, by0 = 0, bh = 1, cells = 0;
+    if constexpr (D == 2 && HAVES) {
+        if (box && gdata) {  // uniform over the block
+            if (threadIdx.x == 0) {
+                s_bb[0] = INT_MAX;
+                s_bb[1] = INT_MIN;
+                s_bb[2] = INT_MAX;
+                s_bb[3] = INT_MIN;
+            }
+            __syncthreads();
+            int lo0 = INT_MAX, hi0 = INT_MIN, lo1 = INT_MAX, hi1 = INT_MIN;
+            for (int r = 0; r < L.ppt; ++r) {
+                const int t = tile_t(L, s, tile, r);
+                if (t < 0) continue;
+                const float y[2] = {sv[t], sv[(size_t)L.nP + t]};
+                int i0[3], i1[3];
+                float fd[3];
+                corners<2>(y, im, i0, i1, fd);
+                lo0 = min(lo0, i0[0]);
+                hi0 = max(hi0, i1[0]);
+                lo1 = min(lo1, i0[1]);
+                hi1 = max(hi1, i1[1]);
+            }
+            lo0 = wave_min(lo0);
+            hi0 = wave_max(hi0);
+            lo1 = wave_min(lo1);
+            hi1 = wave_max(hi1);
+            if (__lane_id() == 0 && lo0 != INT_MAX) {
+                atomicMin(&s_bb[0], lo0);
+                atomicMax(&s_bb[1], hi0);
+                atomicMin(&s_bb[2], lo1);
+                atomicMax(&s_bb[3], hi1);
+            }
+            __syncthreads();
+            bx0 = s_bb[0];
+            by0 = s_bb[2];
+            bh = s_bb[3] - by0 + 1;
+            cells = (s_bb[1] - bx0 + 1) * bh;
+            use_box = bx0 != INT_MAX && (long long)cells * im.C <= box_cap;
+            __syncthreads();  // s_bb is reused by the next tile of a persistent block
+            if (use_box) {
+                for (int i = threadIdx.x; i < cells * im.C; i += BLOCK) box[i] = 0.f;
+                __syncthreads();
+            }
+        }
+    }
     for (int r = 0; r < L.ppt; ++r) {
-        const int t = (tile * L.ppt + r) * BLOCK + threadIdx.x;
-        const bool valid = t < L.nP;
+        const int t = tile_t(L, s, tile, r);
+        const bool valid = t >= 0;
         float q0[D], mu[D];
         int c0 = 0;
 #pragma unroll
@@ -565,8 +663,12 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             corners<D>(y, im, i0, i1, fd);
             for (int c = 0; c < im.C; ++c) {
                 const size_t base = (size_t)c * dstride;
-                interp_backward<D>(data + base, gdata ? gdata + base : nullptr, im, i0, i1, fd,
-                                   gout[(size_t)c * L.nP + t], dfd);
+                const float gv = gout[(size_t)c * L.nP + t];
+                interp_backward<D>(data + base, (gdata && !use_box) ? gdata + base : nullptr, im, i0, i1, fd, gv,
+                                   dfd);
+                if constexpr (D == 2) {
+                    if (use_box) box_add2(box + c * cells, bh, bx0, by0, i0, i1, fd, gv);
+                }
             }
             float w[D];
 #pragma unroll
@@ -581,6 +683,16 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
         }
         pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc);
     }
+    if (use_box) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < cells * im.C; i += BLOCK) {
+            const float v = box[i];
+            if (v != 0.f) {
+                const int c = i / cells, rem = i - c * cells, rr = rem / bh, cc = rem - rr * bh;
+                atomicAdd(gdata + (size_t)c * dstride + (size_t)(bx0 + rr) * im.s1 + (by0 + cc), v);
+            }
+        }
+    }
 }
 
 // adjoint of k_transform_data.  gout [n_theta, C, outsize] in output order;
@@ -589,7 +701,8 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
 template <int D, bool HAVES>
 __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
                                                               const float* __restrict__ gout,
-                                                              const float* __restrict__ saved, float* __restrict__ part) {
+                                                              const float* __restrict__ saved, float* __restrict__ part,
+                                                              int box_cap) {
     extern __shared__ float smem[];
     constexpr int P = D * (D + 1);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -603,7 +716,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
     fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
                              im.gdata ? im.gdata + (size_t)th * im.C * dstride : nullptr,
                              gout + (size_t)th * im.C * L.nP, HAVES ? saved + (size_t)th * (D + D * D) * L.nP : nullptr,
-                             acc + (threadIdx.x >> 6) * nC * P);
+                             acc + (threadIdx.x >> 6) * nC * P, box_cap ? acc + WAVES * nC * P : nullptr, box_cap);
     write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
 }
 
@@ -658,24 +771,40 @@ __global__ __launch_bounds__(BLOCK) void k_interp_bwd(int N, int nP, Src s, Img 
 __global__ __launch_bounds__(BLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
                                                         int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
                                                         float* __restrict__ gtheta, float* __restrict__ gA) {
-    extern __shared__ float sg[];
+    extern __shared__ float sg[];  // [WAVES + 1, Dt]
     const int th = blockIdx.x;
     const int P = ndim * (ndim + 1);
     const int nc[3] = {nc0, nc1, nc2};
     const float* pp = part + (size_t)th * tiles * Dt;
+    // wave w sums tiles w, w + WAVES, ... (two chains for overlap); the wave
+    // sums are then added in wave order
+    const int w = threadIdx.x >> 6;
+    for (int i = threadIdx.x & 63; i < Dt; i += 64) {
+        float s0 = 0.f, s1 = 0.f;
+        int t = w;
+        for (; t + WAVES < tiles; t += 2 * WAVES) {
+            s0 += pp[(size_t)t * Dt + i];
+            s1 += pp[(size_t)(t + WAVES) * Dt + i];
+        }
+        if (t < tiles) s0 += pp[(size_t)t * Dt + i];
+        sg[w * Dt + i] = s0 + s1;
+    }
+    __syncthreads();
+    float* sA = sg + WAVES * Dt;
     for (int i = threadIdx.x; i < Dt; i += BLOCK) {
-        float s = 0.f;
-        for (int t = 0; t < tiles; ++t) s += pp[(size_t)t * Dt + i];
+        float s = sg[i];
+#pragma unroll
+        for (int k = 1; k < WAVES; ++k) s += sg[k * Dt + i];
         const int rem = i % P, r = rem / (ndim + 1), col = rem % (ndim + 1);
         const float sc = (col < ndim) ? (float)nc[r] / (float)nc[col] : (float)nc[r];
         s *= sc;
-        sg[i] = s;
+        sA[i] = s;
         if (gA) gA[(size_t)th * Dt + i] = s;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < d; k += BLOCK) {
         float s = 0.f;
-        for (int i = 0; i < Dt; ++i) s = fmaf(B[(size_t)i * d + k], sg[i], s);
+        for (int i = 0; i < Dt; ++i) s = fmaf(B[(size_t)i * d + k], sA[i], s);
         gtheta[(size_t)th * d + k] = s;
     }
 }
@@ -729,6 +858,7 @@ __device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, flo
     L.nP = I->nP;
     L.tiles = I->tiles;
     L.ppt = ppt;
+    L.ty = tile_ty(nd, nd > 1 ? I->outsize[1] : 1);
     L.t_end = t_end;
     g.ndim = nd;
     g.nc0 = I->tess.nc[0];
@@ -829,8 +959,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __re
 }
 
 template <int D, bool HAVES>
-__device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, const cpab_instance_t* I, bool restage,
-                                               const Launch& L, const Geo& g, const Src& s, const Img& im, int tile) {
+__device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, float* box, int box_cap,
+                                               const cpab_instance_t* I, bool restage, const Launch& L, const Geo& g,
+                                               const Src& s, const Img& im, int tile) {
     constexpr int P = D * (D + 1);
     const int nCP = g.nC * P;
     float* acc = acc_base;
@@ -838,7 +969,7 @@ __device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, con
     const CellTable<D> tab = restage ? stage_tables<D>(smem, static_cast<const float*>(I->ws), g) : table_view<D>(smem, g.nC);
     if (!restage) __syncthreads();
     fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data, im.gdata, I->grad_out, I->saved,
-                             acc + (threadIdx.x >> 6) * nCP);
+                             acc + (threadIdx.x >> 6) * nCP, box, box_cap);
     float* part = reinterpret_cast<float*>(static_cast<char*>(I->ws) + align256_hd((size_t)nCP * 4));
     write_partials<D>(acc, part + (size_t)tile * nCP, nCP);
 }
@@ -846,7 +977,8 @@ __device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, con
 template <bool HAVES>
 __global__ __launch_bounds__(BLOCK) void k_batch_bwd(const cpab_instance_t* __restrict__ inst,
                                                      const int* __restrict__ item_inst, int n_items, int ppt,
-                                                     float t_end, int* counter, int acc_offset) {
+                                                     float t_end, int* counter, int acc_offset, int box_offset,
+                                                     int box_cap) {
     extern __shared__ float smem[];
     __shared__ int s_item;
     int cur = -1;
@@ -864,10 +996,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_bwd(const cpab_instance_t* __re
         const bool restage = ii != cur;
         cur = ii;
         float* acc = smem + acc_offset;  // after the largest table of the batch
+        float* box = box_cap ? smem + box_offset : nullptr;
         switch (g.ndim) {
-            case 1: batch_bwd_item<1, HAVES>(smem, acc, I, restage, L, g, s, im, tile); break;
-            case 2: batch_bwd_item<2, HAVES>(smem, acc, I, restage, L, g, s, im, tile); break;
-            default: batch_bwd_item<3, HAVES>(smem, acc, I, restage, L, g, s, im, tile); break;
+            case 1: batch_bwd_item<1, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
+            case 2: batch_bwd_item<2, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
+            default: batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
         }
     }
 }
@@ -1096,6 +1229,16 @@ int pick_ppt(int n_theta, int nP) {
     return 1;
 }
 
+// the fused backward of 2D tiled grids runs 512-point tiles (16 x 32 for the
+// usual grids) whose data footprint fits the LDS box
+int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
+    const int ppt = pick_ppt(n_theta, nP);
+    return (tile_ty(ndim, g1) > 0 && ppt > 2) ? 2 : ppt;
+}
+
+// LDS floats of the dL/ddata box of the fused 2D backward
+constexpr int BOX_FLOATS = 4096;
+
 size_t lds_tables(int ndim, int nC) {
     switch (ndim) {
         case 1: return table_floats<1>(nC) * 4;
@@ -1128,8 +1271,9 @@ int launch_reduce(const cpab_tess_t* t, const float* part, int n_theta, int tile
                   hipStream_t st) {
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
     Geo g = make_geo(t);
-    hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(BLOCK), Dt * sizeof(float), st, part, tiles, Dt, t->ndim,
-                       g.nc0, g.nc1, g.nc2, t->B, t->d, gtheta, gA);
+    if ((size_t)(WAVES + 1) * Dt * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
+    hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(BLOCK), (WAVES + 1) * Dt * sizeof(float), st, part, tiles,
+                       Dt, t->ndim, g.nc0, g.nc1, g.nc2, t->B, t->d, gtheta, gA);
     return status_from_hip();
 }
 
@@ -1195,6 +1339,7 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     L.n_theta = n_theta;
     L.nP = nP;
     L.ppt = pick_ppt(n_theta, nP);
+    L.ty = 0;
     L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -1230,6 +1375,7 @@ int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     L.n_theta = n_theta;
     L.nP = nP;
     L.ppt = pick_ppt(n_theta, nP);
+    L.ty = 0;
     L.tiles = nP > 0 ? (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt) : 1;
     L.t_end = t_end;
     Geo g = make_geo(t);
@@ -1314,6 +1460,16 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
 
 size_t cpab_saved_floats(int ndim, int n_theta, int nP) { return (size_t)n_theta * (ndim + ndim * ndim) * nP; }
 
+size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* outsize) {
+    if (check_tess(t) != CPAB_OK || n_theta < 0 || !outsize) return 0;
+    Src s = {};
+    int nP;
+    if (src_out(t->ndim, outsize, s, nP)) return 0;
+    const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
+    const size_t tiles = tile_count(t->ndim, nP, s.g0, s.g1, pick_ppt_bwd(t->ndim, n_theta, nP, s.g1));
+    return align256((size_t)n_theta * Dt * 4) + align256((size_t)n_theta * tiles * Dt * 4);
+}
+
 int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
                             const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
                             float* saved, void* workspace, size_t ws_bytes, hipStream_t stream) {
@@ -1326,17 +1482,18 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     if (!theta || !lin || !out || n_theta < 0) return CPAB_ERR_ARG;
     if (n_theta == 0) return CPAB_OK;
-    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    if (ws_bytes < cpab_data_workspace_bytes(t, n_theta, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
     L.ppt = pick_ppt(n_theta, nP);
+    L.ty = tile_ty(t->ndim, s.g1);
     const size_t lds = lds_tables(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     s.lin = lin;
-    L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
+    L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
     dim3 grid(L.tiles * n_theta);
@@ -1363,12 +1520,15 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     if (!theta || !lin || !grad_out || !grad_theta || n_theta < 0) return CPAB_ERR_ARG;
     if (n_theta == 0) return CPAB_OK;
-    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    if (ws_bytes < cpab_data_workspace_bytes(t, n_theta, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
-    L.ppt = pick_ppt(n_theta, nP);
-    const size_t lds = lds_bwd(t->ndim, t->nC);
+    L.ppt = pick_ppt_bwd(t->ndim, n_theta, nP, s.g1);
+    L.ty = tile_ty(t->ndim, s.g1);
+    // 2D with saved state and dL/ddata: accumulate dL/ddata per tile in LDS
+    const int box_cap = (t->ndim == 2 && L.ty > 0 && saved && grad_data) ? BOX_FLOATS : 0;
+    const size_t lds = lds_bwd(t->ndim, t->nC) + (size_t)box_cap * 4;
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
     float* Aq = static_cast<float*>(workspace);
@@ -1376,17 +1536,17 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     im.gdata = grad_data;
     s.lin = lin;
-    L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
+    L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt);
     L.t_end = t_end;
     Geo g = make_geo(t);
     dim3 grid(L.tiles * n_theta);
 #define LAUNCH_DB(DD)                                                                                                     \
     if (saved)                                                                                                             \
         hipLaunchKernelGGL((k_transform_data_bwd<DD, true>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, grad_out,    \
-                           saved, part);                                                                                   \
+                           saved, part, box_cap);                                                                          \
     else                                                                                                                   \
         hipLaunchKernelGGL((k_transform_data_bwd<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, grad_out,   \
-                           saved, part);
+                           saved, part, 0);
     CPAB_DISPATCH_D(t->ndim, LAUNCH_DB)
 #undef LAUNCH_DB
     if ((rc = status_from_hip())) return rc;
@@ -1436,6 +1596,7 @@ int cpab_transform_data_fwd_plain(const cpab_tess_t* t, const float* theta, int 
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     const size_t dstride = (size_t)im.s0 * im.s1 * im.s2;
     const size_t sv = (size_t)(t->ndim + t->ndim * t->ndim) * nP;
+    if (ws_bytes < cpab_data_workspace_bytes(t, 1, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
     for (int i = 0; i < n_theta; ++i) {
         rc = cpab_transform_data_fwd(t, theta + (size_t)i * t->d, 1, data + (size_t)i * C * dstride, C, dsize, outsize,
                                      lin, t_end, out + (size_t)i * C * nP, saved ? saved + i * sv : nullptr, workspace,
@@ -1458,6 +1619,7 @@ int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int 
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     const size_t dstride = (size_t)im.s0 * im.s1 * im.s2;
     const size_t sv = (size_t)(t->ndim + t->ndim * t->ndim) * nP;
+    if (ws_bytes < cpab_data_workspace_bytes(t, 1, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
     for (int i = 0; i < n_theta; ++i) {
         rc = cpab_transform_data_bwd(t, theta + (size_t)i * t->d, 1, data + (size_t)i * C * dstride, C, dsize, outsize,
                                      lin, t_end, grad_out + (size_t)i * C * nP, saved ? saved + i * sv : nullptr,
@@ -1468,26 +1630,36 @@ int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int 
     return CPAB_OK;
 }
 
-size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, int nP, int ppt) {
-    if (check_tess(t) != CPAB_OK || nP < 0 || ppt < 1) return 0;
+size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, const int* outsize, int ppt) {
+    if (check_tess(t) != CPAB_OK || !outsize || ppt < 1) return 0;
+    Src s = {};
+    int nP;
+    if (src_out(t->ndim, outsize, s, nP)) return 0;
     const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
-    size_t tiles = (nP + (size_t)BLOCK * ppt - 1) / ((size_t)BLOCK * ppt);
-    if (tiles == 0) tiles = 1;
+    const size_t tiles = tile_count(t->ndim, nP, s.g0, s.g1, ppt);
     const size_t batched = align256(Dt * 4) + align256(tiles * Dt * 4);
-    const size_t plain = cpab_workspace_bytes(t, 1, nP);
+    const size_t plain = cpab_data_workspace_bytes(t, 1, outsize);
     return batched > plain ? batched : plain;
 }
 
 int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, int max_items) {
-    if (!inst || n_inst < 0 || ppt < 1 || !item_inst) return -CPAB_ERR_ARG;
+    if (!inst || n_inst < 0 || ppt < 1) return -CPAB_ERR_ARG;
     long long item = 0;
     for (int i = 0; i < n_inst; ++i) {
-        cpab_instance_t& I = inst[i];
+        const cpab_instance_t& I = inst[i];
         int rc = check_tess(&I.tess);
         if (rc) return -rc;
-        int tiles = (I.nP + BLOCK * ppt - 1) / (BLOCK * ppt);
-        if (tiles < 1) tiles = 1;
-        if (item + tiles > max_items) return -CPAB_ERR_WORKSPACE;
+        const int nd = I.tess.ndim;
+        item += tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt);
+        if (item > INT_MAX) return -CPAB_ERR_UNSUPPORTED;
+    }
+    if (!item_inst) return (int)item;  // count only
+    if (item > max_items) return -CPAB_ERR_WORKSPACE;
+    item = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        cpab_instance_t& I = inst[i];
+        const int nd = I.tess.ndim;
+        const int tiles = tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt);
         I.item0 = (int)item;
         I.tiles = tiles;
         for (int k = 0; k < tiles; ++k) item_inst[item + k] = i;
@@ -1508,7 +1680,7 @@ static int batch_check(const cpab_instance_t* h, int n_inst, int ppt, size_t& ld
         if (rc) return rc;
         if (!I.theta || !I.data || !I.lin || !I.out || !I.ws) return CPAB_ERR_ARG;
         if (bwd && (!I.grad_out || !I.grad_theta)) return CPAB_ERR_ARG;
-        if (I.ws_bytes < cpab_batch_workspace_bytes(&I.tess, I.nP, ppt)) return CPAB_ERR_WORKSPACE;
+        if (I.ws_bytes < cpab_batch_workspace_bytes(&I.tess, I.outsize, ppt)) return CPAB_ERR_WORKSPACE;
         const int nd = I.tess.ndim;
         const size_t Dt = (size_t)I.tess.nC * nd * (nd + 1);
         const size_t tb = lds_tables(nd, I.tess.nC);
@@ -1562,12 +1734,21 @@ int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
                        dev_inst);
     hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, stream, counter);
     const int acc_off = (int)(lds_tab / 4);
+    // LDS box for dL/ddata when some 2D instance has a tiled grid and wants it
+    bool want_box = false;
+    for (int i = 0; i < n_inst; ++i) {
+        const cpab_instance_t& I = host_inst[i];
+        want_box = want_box || (I.tess.ndim == 2 && I.grad_data && tile_ty(2, I.outsize[1]) > 0);
+    }
+    const int box_cap = (have && want_box && lds_tab + lds_acc + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT) ? BOX_FLOATS : 0;
+    const int box_off = (int)((lds_tab + lds_acc) / 4);
+    const size_t lds = lds_tab + lds_acc + (size_t)box_cap * 4;
     if (have)
-        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(blocks), dim3(BLOCK), lds_tab + lds_acc, stream, dev_inst,
-                           dev_item_inst, n_items, ppt, t_end, counter, acc_off);
+        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst, n_items,
+                           ppt, t_end, counter, acc_off, box_off, box_cap);
     else
-        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(blocks), dim3(BLOCK), lds_tab + lds_acc, stream, dev_inst,
-                           dev_item_inst, n_items, ppt, t_end, counter, acc_off);
+        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst,
+                           n_items, ppt, t_end, counter, acc_off, box_off, 0);
     if ((rc = status_from_hip())) return rc;
     hipLaunchKernelGGL(k_batch_reduce, dim3(n_inst), dim3(BLOCK), maxDt * sizeof(float), stream, dev_inst);
     return status_from_hip();

@@ -44,8 +44,13 @@ class DeviceTess:
         self.struct = t
         self.ref = ctypes.byref(t)
 
-    def workspace(self, n_theta, nP):
-        nbytes = int(N.lib().cpab_workspace_bytes(self.ref, n_theta, nP))
+    def workspace(self, n_theta, nP, outsize=None):
+        """Workspace of a transform call on n_theta x nP points, or (outsize
+        given) of a transform_data call onto that meshgrid."""
+        if outsize is None:
+            nbytes = int(N.lib().cpab_workspace_bytes(self.ref, n_theta, nP))
+        else:
+            nbytes = int(N.lib().cpab_data_workspace_bytes(self.ref, n_theta, _int_arr(outsize)))
         ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
         return ws, nbytes
 
@@ -209,12 +214,10 @@ class TransformData(torch.autograd.Function):
             saved = torch.empty(int(N.lib().cpab_saved_floats(dev.ndim, n_theta, nP)), device=theta.device,
                                 dtype=torch.float32)
         plain = getattr(params, "launch", "batched") == "plain"
-        ws, nb = dev.workspace(1 if plain else n_theta, nP)
+        ws, nb = dev.workspace(1 if plain else n_theta, nP, outsize)
         fn = N.lib().cpab_transform_data_fwd_plain if plain else N.lib().cpab_transform_data_fwd
-        N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C,
-                                                _int_arr(data.shape[2:]), _int_arr(outsize), _ptr(lin),
-                                                float(t_end), _ptr(out), _ptr(saved), _ptr(ws), nb,
-                                                _stream(theta.device)))
+        N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C, _int_arr(data.shape[2:]), _int_arr(outsize),
+                   _ptr(lin), float(t_end), _ptr(out), _ptr(saved), _ptr(ws), nb, _stream(theta.device)))
         ctx.params = params
         ctx.outsize = tuple(outsize)
         ctx.t_end = t_end
@@ -233,10 +236,9 @@ class TransformData(torch.autograd.Function):
         gdata = torch.zeros_like(data) if ctx.needs_input_grad[0] else None
         gtheta = torch.empty(n_theta, dev.d, device=theta.device, dtype=torch.float32)
         plain = getattr(ctx.params, "launch", "batched") == "plain"
-        ws, nb = dev.workspace(1 if plain else n_theta, nP)
+        ws, nb = dev.workspace(1 if plain else n_theta, nP, ctx.outsize)
         fn = N.lib().cpab_transform_data_bwd_plain if plain else N.lib().cpab_transform_data_bwd
-        N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C,
-                                                _int_arr(data.shape[2:]), _int_arr(ctx.outsize), _ptr(lin),
-                                                float(ctx.t_end), _ptr(grad), _ptr(saved), _ptr(gdata), _ptr(gtheta),
-                                                _ptr(ws), nb, _stream(theta.device)))
+        N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C, _int_arr(data.shape[2:]), _int_arr(ctx.outsize),
+                   _ptr(lin), float(ctx.t_end), _ptr(grad), _ptr(saved), _ptr(gdata), _ptr(gtheta), _ptr(ws), nb,
+                   _stream(theta.device)))
         return gdata, (gtheta if ctx.needs_input_grad[1] else None), None, None, None

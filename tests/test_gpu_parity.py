@@ -151,6 +151,31 @@ def test_transform_data_fused_matches_unfused(cuda, name, n_theta, scale, size, 
     assert np.abs(out1.detach().cpu().numpy() - ref).max() <= POS_TOL * max(size) * 2 + 1e-6
 
 
+@pytest.mark.parametrize("dsize,outsize", [([512, 512], [64, 64]), ([40, 50], [100, 90]), ([33, 7], [50, 5]),
+                                           ([64, 64], [64, 64])])
+def test_transform_data_2d_tiles_and_lds_box(cuda, dsize, outsize):
+    """The 2D fused backward accumulates dL/ddata of each output tile in an LDS
+    box when its footprint fits (downsampling overflows it: global atomics;
+    a second axis below 8 runs untiled).  Gradients equal the unfused path's."""
+    T, b = make_cpab("2d_4x4")
+    theta = _theta(T.get_theta_dim(), 3, 0.3, 21)
+    g = torch.Generator().manual_seed(22)
+    data = torch.rand([3, 3] + dsize, generator=g)
+    gout = torch.randn([3, 3] + outsize, generator=g)
+    th1 = theta.to(cuda).requires_grad_(True)
+    d1 = data.to(cuda).requires_grad_(True)
+    out1 = T.transform_data(d1, th1, outsize)
+    (out1 * gout.to(cuda)).sum().backward()
+    th2 = theta.to(cuda).requires_grad_(True)
+    d2 = data.to(cuda).requires_grad_(True)
+    gt = T.transform_grid(T.uniform_meshgrid(outsize), th2)
+    out2 = T.interpolate(d2, gt, outsize)
+    (out2 * gout.to(cuda)).sum().backward()
+    assert torch.allclose(out1, out2, atol=1e-6, rtol=0)
+    assert torch.allclose(d1.grad, d2.grad, atol=1e-5, rtol=0)
+    assert _rel(th1.grad.cpu().double().numpy(), th2.grad.cpu().double().numpy()) <= 1e-4
+
+
 def test_identity_theta_is_exact_identity(cuda):
     T, b = make_cpab("2d_4x4")
     grid = T.uniform_meshgrid([64, 48])

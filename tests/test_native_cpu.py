@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = N.lib()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.cpab_abi_version() == 1
+    assert lib.cpab_abi_version() == 2
     assert lib.cpab_status_string(0) == b"ok"
     assert b"workspace" in lib.cpab_status_string(3)
 
@@ -81,6 +81,13 @@ def test_workspace_and_saved_sizes():
     ws = L.cpab_workspace_bytes(ctypes.byref(t), 256, 65536)
     assert ws >= 256 * D * 4
     assert L.cpab_saved_floats(2, 256, 65536) == 256 * 6 * 65536
+    # transform_data: A_q plus one dL/dA partial per 2D tile (16 x 32 points in the backward)
+    osz = (ctypes.c_int * 3)(256, 256, 1)
+    wd = L.cpab_data_workspace_bytes(ctypes.byref(t), 256, osz)
+    assert wd == 256 * D * 4 + 256 * (256 // 16) * (256 // 32) * D * 4
+    # ragged grids still get whole tiles
+    osz = (ctypes.c_int * 3)(100, 37, 1)
+    assert L.cpab_batch_workspace_bytes(ctypes.byref(t), osz, 1) >= 256 + (100 + 7) // 8 * 2 * D * 4
 
 
 def test_cpab_facade_contract():
