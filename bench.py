@@ -414,6 +414,21 @@ def run_ragged(args):
     print(json.dumps(line), flush=True)
 
 
+BWD_KERNEL = "k_transform_data_bwd<2, true>"
+TRAFFIC_FILE = "profiles/images2d_k_transform_data.traffic.json"
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    1 GiB copy calibration) from the committed rocprofv3 counter summary of
+    this workload, or None."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -494,7 +509,8 @@ def main():
         "max_err_fwd": res.get("max_err_fwd"),
         "max_rel_err_grad": res.get("max_rel_err_grad"),
         "roofline": {"kernel": "k_transform_data_bwd", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(BWD_KERNEL),
+                     "traffic_source": TRAFFIC_FILE,
                      "algorithmic_bytes_per_launch": algo_bytes, "launch_ms": bwd_ms},
         "cpu_baseline": cpu,
     }
