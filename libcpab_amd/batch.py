@@ -207,8 +207,11 @@ class CpabBatch:
         for ppt in PPT_CHOICES:
             for bpc in BLOCKS_PER_CU:
                 self.set_launch(ppt, bpc * cus)
-                self.forward()
-                self.backward()
+                try:
+                    self.forward()
+                    self.backward()
+                except N.CpabError:  # this ppt does not fit the LDS budget of the batch
+                    continue
                 torch.cuda.synchronize(self.device)
                 t0 = time.perf_counter()
                 for _ in range(reps):
@@ -219,6 +222,8 @@ class CpabBatch:
                 results.append((dt, ppt, bpc * cus))
                 if best is None or dt < best[0]:
                     best = (dt, ppt, bpc * cus)
+        if best is None:
+            raise N.CpabError("no launch configuration of this batch fits the LDS budget")
         if verbose:
             for r in sorted(results):
                 print("autotune: %.3f ms ppt=%d blocks=%d" % (r[0] * 1e3, r[1], r[2]))
