@@ -273,7 +273,7 @@ CPAB_HD __forceinline__ void expm_apply(const float (&M)[D][D], float s, const f
     for (int a = 0; a < D; ++a) r[a] = u[a];
 #pragma unroll
     for (int k = KS; k >= 1; --k) {
-        const float f = s / (float)k;
+        const float f = s * (1.0f / (float)k);
         float t[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) {
@@ -341,8 +341,9 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
     // Bounds on the chunk (|n| = 1, |M| <= |M|_F, |M|_F h <= RHO, |cf_k| <=
     // |M|^(k-1) |v0| / k!):  |G''(s)| <= K = |M|_F |v0| e^RHO and
     // |G''(s) - 2 cf2 - 6 cf3 s| <= K4 = |M|_F^3 |v0| h^2 e^RHO / 2.
-    const float mh = h / tab.inv_n[c];
-    const float K = GROW * (vn / tab.inv_n[c]);
+    const float nrm = fast_rcp(tab.inv_n[c]);  // |M|_F (the bounds below carry margin for its last bit)
+    const float mh = h * nrm;
+    const float K = GROW * vn * nrm;
     const float K4 = 0.5f * K * mh * mh;
     const float k2h2 = 0.5f * K * h * h;
 

@@ -387,6 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
     const int nC = g.nC;
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * D * (D + 1), g);
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
     for (int r = 0; r < L.ppt; ++r) {
         const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
         integrate<D, WITHJ>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
         float* o = out + (size_t)th * D * L.nP;
 #pragma unroll
-        for (int a = 0; a < D; ++a) o[(size_t)a * L.nP + p] = q[a] / (float)nc[a];
+        for (int a = 0; a < D; ++a) o[(size_t)a * L.nP + p] = q[a] * rnc[a];
         if constexpr (WITHJ) {
             float* jo = jac + (size_t)th * D * D * L.nP;
 #pragma unroll
@@ -422,6 +423,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
                                                const CellTable<D>& tab, int tile, const float* __restrict__ data,
                                                float* __restrict__ out, float* __restrict__ sv) {
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
     for (int r = 0; r < L.ppt; ++r) {
@@ -440,7 +442,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
         integrate<D, SAVE>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
         float y[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) y[a] = q[a] / (float)nc[a];
+        for (int a = 0; a < D; ++a) y[a] = q[a] * rnc[a];
         if constexpr (SAVE) {
             // indexed in output order t (the order threads run in) for coalesced stores
 #pragma unroll
@@ -500,6 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
     float* wacc = acc + (threadIdx.x >> 6) * nC * P;
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
     for (int r = 0; r < L.ppt; ++r) {
         const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
@@ -535,7 +538,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
             float w[D];
             const float* go = gout + (size_t)th * D * L.nP;
 #pragma unroll
-            for (int a = 0; a < D; ++a) w[a] = go[(size_t)a * L.nP + p] / (float)nc[a];
+            for (int a = 0; a < D; ++a) w[a] = go[(size_t)a * L.nP + p] * rnc[a];
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 float acc2 = 0.f;
@@ -570,6 +573,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
                                                float* __restrict__ gdata, const float* __restrict__ gout,
                                                const float* __restrict__ sv, float* wacc, float* box, int box_cap) {
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const int sz[3] = {im.s0, im.s1, im.s2};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
@@ -656,7 +660,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
                 for (int a = 0; a < D; ++a) q[a] = q0[a];
                 integrate<D, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
 #pragma unroll
-                for (int a = 0; a < D; ++a) y[a] = q[a] / (float)nc[a];
+                for (int a = 0; a < D; ++a) y[a] = q[a] * rnc[a];
             }
             int i0[3], i1[3];
             float fd[3], dfd[3] = {0.f, 0.f, 0.f};
@@ -672,7 +676,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             }
             float w[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) w[a] = dfd[a] * (float)(sz[a] - 1) / (float)nc[a];
+            for (int a = 0; a < D; ++a) w[a] = dfd[a] * (float)(sz[a] - 1) * rnc[a];
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 float acc2 = 0.f;
@@ -829,6 +833,7 @@ __global__ __launch_bounds__(BLOCK) void k_vectorfield(Geo g, const float* __res
     const int p = blockIdx.x * BLOCK + threadIdx.x;
     if (p >= nP) return;
     const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     float x[D], q[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -842,7 +847,7 @@ __global__ __launch_bounds__(BLOCK) void k_vectorfield(Geo g, const float* __res
         float acc = A[r * (D + 1) + D];
 #pragma unroll
         for (int k = 0; k < D; ++k) acc = fmaf(A[r * (D + 1) + k], q[k], acc);
-        v[(size_t)r * nP + p] = acc / (float)nc[r];
+        v[(size_t)r * nP + p] = acc * rnc[r];
     }
 }
 
