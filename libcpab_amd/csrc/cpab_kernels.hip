@@ -1254,7 +1254,8 @@ size_t lds_tables(int ndim, int nC) {
 
 size_t lds_bwd(int ndim, int nC) { return lds_tables(ndim, nC) + (size_t)WAVES * nC * ndim * (ndim + 1) * 4; }
 
-constexpr size_t LDS_LIMIT = 64 * 1024;
+// LDS one workgroup may allocate on gfx950 (160 KB per CU, all of it per block)
+constexpr size_t LDS_LIMIT = 160 * 1024;
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -1532,7 +1533,10 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.ppt = pick_ppt_bwd(t->ndim, n_theta, nP, s.g1);
     L.ty = tile_ty(t->ndim, s.g1);
     // 2D with saved state and dL/ddata: accumulate dL/ddata per tile in LDS
-    const int box_cap = (t->ndim == 2 && L.ty > 0 && saved && grad_data) ? BOX_FLOATS : 0;
+    const int box_cap = (t->ndim == 2 && L.ty > 0 && saved && grad_data &&
+                         lds_bwd(t->ndim, t->nC) + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT)
+                            ? BOX_FLOATS
+                            : 0;
     const size_t lds = lds_bwd(t->ndim, t->nC) + (size_t)box_cap * 4;
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
