@@ -876,6 +876,22 @@ __device__ __forceinline__ float wave_total(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Inclusive prefix sum over the 64 lanes (lane i gets lanes 0..i), same DPP
+// pattern as wave_total.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_take_int(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, true);
+}
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    v += dpp_take_int<0x111, 0xF>(v);
+    v += dpp_take_int<0x112, 0xF>(v);
+    v += dpp_take_int<0x114, 0xF>(v);
+    v += dpp_take_int<0x118, 0xF>(v);
+    v += dpp_take_int<0x142, 0xA>(v);
+    v += dpp_take_int<0x143, 0xC>(v);
+    return v;
+}
+
 // Minimum / maximum over the 64 lanes (same DPP pattern; lanes without a
 // source keep the identity), read into a scalar.
 template <int CTRL, int ROWS>

@@ -74,6 +74,31 @@ __host__ __device__ inline int tile_count(int ndim, int nP, int g0, int g1, int 
     return ((g0 + tx - 1) / tx) * ((g1 + ty - 1) / ty);
 }
 
+// dL/ddata bins of a 2D tile: at most BIN_CELLS footprint cells (11-bit ids)
+constexpr int BIN_CELLS = 2048;
+
+// in-place exclusive prefix sum of v[0, n) by the whole block; v[n-1] becomes
+// the sum of the first n-1 entries
+__device__ __forceinline__ void block_exclusive_scan(int* v, int n) {
+    __shared__ int s_wsum[WAVES];
+    const int per = (n + BLOCK - 1) / BLOCK;
+    const int b0 = (int)threadIdx.x * per, b1 = min(b0 + per, n);
+    int loc = 0;
+    for (int k = b0; k < b1; ++k) loc += v[k];
+    const int inc = wave_inclusive_scan(loc);
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 63) s_wsum[w] = inc;
+    __syncthreads();
+    int run = inc - loc;
+    for (int w2 = 0; w2 < w; ++w2) run += s_wsum[w2];
+    for (int k = b0; k < b1; ++k) {
+        const int x = v[k];
+        v[k] = run;
+        run += x;
+    }
+    __syncthreads();
+}
+
 // output-order index t of round r of this thread in tile `tile`, -1 if outside
 __device__ __forceinline__ int tile_t(const Launch& L, const Src& s, int tile, int r) {
     if (L.ty == 0) {
@@ -264,18 +289,6 @@ __device__ __forceinline__ float interp_value(const float* __restrict__ dp, cons
         const float c1 = c01 * (1.f - fd[1]) + c11 * fd[1];
         return c0 * (1.f - fd[2]) + c1 * fd[2];
     }
-}
-
-// interp_backward's dL/ddata of one channel (2D) into an LDS box whose row r,
-// column k holds data cell (bx0 + r, by0 + k)
-__device__ __forceinline__ void box_add2(float* b, int bh, int bx0, int by0, const int (&i0)[3], const int (&i1)[3],
-                                         const float (&fd)[3], float g) {
-    const float wx0 = 1.f - fd[0], wx1 = fd[0], wy0 = 1.f - fd[1], wy1 = fd[1];
-    const int r0 = (i0[0] - bx0) * bh, r1 = (i1[0] - bx0) * bh, k0 = i0[1] - by0, k1 = i1[1] - by0;
-    atomicAdd(b + r0 + k0, g * wx0 * wy0);
-    atomicAdd(b + r1 + k0, g * wx1 * wy0);
-    atomicAdd(b + r0 + k1, g * wx0 * wy1);
-    atomicAdd(b + r1 + k1, g * wx1 * wy1);
 }
 
 template <int D>
@@ -563,10 +576,13 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
 // One tile of one instance's fused backward; adds dL/dA_q of the tile's points
 // into this wave's LDS accumulator wacc (all lanes of the block take part).
 //
-// dL/ddata of a 2D tile with saved state goes through an LDS box when box
-// (gbox_cap floats, nullable) holds the bounding box of the tile's bilinear
-// footprint for all channels: lanes add into LDS, and the box is flushed with
-// one global atomic per touched cell instead of four per point and channel.
+// dL/ddata of a 2D tile with saved state (C <= 4) goes through LDS when box
+// (box_cap words, nullable) holds the tile's footprint bins and point slots:
+// each point is binned by the data cell of its lower corner (one integer LDS
+// atomic), the bins are scanned into a point list, and every cell of the
+// footprint's bounding box gathers the weighted grad_out of the points in the
+// four bins that touch it and adds it to dL/ddata with one global atomic per
+// channel -- instead of four atomics per point and channel.
 template <int D, bool HAVES>
 __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, const Src& s, const Img& im,
                                                const CellTable<D>& tab, int tile, const float* __restrict__ data,
@@ -581,6 +597,12 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     __shared__ int s_bb[4];
     bool use_box = false;
     int bx0 = 0, by0 = 0, bh = 1, cells = 0;
+    const int nslot = L.ppt * BLOCK;
+    int* bins = reinterpret_cast<int*>(box);  // [cells + 1] counts, then offsets
+    int* code = bins;                         // [nslot] cell | rank << 11 | flags
+    int* list = bins;                         // [nslot] slots by bin
+    float* fdv = box;                         // [2, nslot]
+    float* gvv = box;                         // [C, nslot]
     if constexpr (D == 2 && HAVES) {
         if (box && gdata) {  // uniform over the block
             if (threadIdx.x == 0) {
@@ -618,10 +640,15 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             by0 = s_bb[2];
             bh = s_bb[3] - by0 + 1;
             cells = (s_bb[1] - bx0 + 1) * bh;
-            use_box = bx0 != INT_MAX && (long long)cells * im.C <= box_cap;
+            use_box = bx0 != INT_MAX && im.C <= 4 && cells < BIN_CELLS &&
+                      (long long)cells + 1 + (long long)nslot * (4 + im.C) <= box_cap;
             __syncthreads();  // s_bb is reused by the next tile of a persistent block
             if (use_box) {
-                for (int i = threadIdx.x; i < cells * im.C; i += BLOCK) box[i] = 0.f;
+                code = bins + cells + 1;
+                list = code + nslot;
+                fdv = reinterpret_cast<float*>(list + nslot);
+                gvv = fdv + 2 * nslot;
+                for (int i = threadIdx.x; i <= cells; i += BLOCK) bins[i] = 0;
                 __syncthreads();
             }
         }
@@ -629,6 +656,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     for (int r = 0; r < L.ppt; ++r) {
         const int t = tile_t(L, s, tile, r);
         const bool valid = t >= 0;
+        if (use_box && !valid) code[r * BLOCK + threadIdx.x] = -1;
         float q0[D], mu[D];
         int c0 = 0;
 #pragma unroll
@@ -665,13 +693,21 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             int i0[3], i1[3];
             float fd[3], dfd[3] = {0.f, 0.f, 0.f};
             corners<D>(y, im, i0, i1, fd);
+            const int slot = r * BLOCK + threadIdx.x;
             for (int c = 0; c < im.C; ++c) {
                 const size_t base = (size_t)c * dstride;
                 const float gv = gout[(size_t)c * L.nP + t];
                 interp_backward<D>(data + base, (gdata && !use_box) ? gdata + base : nullptr, im, i0, i1, fd, gv,
                                    dfd);
-                if constexpr (D == 2) {
-                    if (use_box) box_add2(box + c * cells, bh, bx0, by0, i0, i1, fd, gv);
+                if (use_box) gvv[c * nslot + slot] = gv;
+            }
+            if constexpr (D == 2) {
+                if (use_box) {
+                    const int k = (i0[0] - bx0) * bh + (i0[1] - by0);
+                    const int rank = atomicAdd(&bins[k], 1);
+                    code[slot] = k | (rank << 11) | ((i1[0] != i0[0]) << 29) | ((i1[1] != i0[1]) << 30);
+                    fdv[slot] = fd[0];
+                    fdv[nslot + slot] = fd[1];
                 }
             }
             float w[D];
@@ -689,12 +725,37 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     }
     if (use_box) {
         __syncthreads();
-        for (int i = threadIdx.x; i < cells * im.C; i += BLOCK) {
-            const float v = box[i];
-            if (v != 0.f) {
-                const int c = i / cells, rem = i - c * cells, rr = rem / bh, cc = rem - rr * bh;
-                atomicAdd(gdata + (size_t)c * dstride + (size_t)(bx0 + rr) * im.s1 + (by0 + cc), v);
-            }
+        block_exclusive_scan(bins, cells + 1);
+        for (int sl = threadIdx.x; sl < nslot; sl += BLOCK) {
+            const int cd = code[sl];
+            if (cd >= 0) list[bins[cd & 2047] + ((cd >> 11) & 0x3FFFF)] = sl;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < cells; k += BLOCK) {
+            const int i = k / bh, j = k - i * bh;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    if (i < a || j < b) continue;
+                    const int kb = k - a * bh - b;
+                    for (int e = bins[kb]; e < bins[kb + 1]; ++e) {
+                        const int sl = list[e];
+                        const int cd = code[sl];
+                        const bool rf = (cd >> 29) & 1, cf = (cd >> 30) & 1;
+                        if ((a && !rf) || (b && !cf)) continue;
+                        const float f0 = fdv[sl], f1 = fdv[nslot + sl];
+                        const float wr = a ? f0 : (rf ? 1.f - f0 : 1.f);
+                        const float wc = b ? f1 : (cf ? 1.f - f1 : 1.f);
+                        const float w = wr * wc;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            if (c < im.C) acc[c] = fmaf(w, gvv[c * nslot + sl], acc[c]);
+                    }
+                }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (c < im.C && acc[c] != 0.f)
+                    atomicAdd(gdata + (size_t)c * dstride + (size_t)(bx0 + i) * im.s1 + (by0 + j), acc[c]);
         }
     }
 }
@@ -1241,8 +1302,9 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
     return (tile_ty(ndim, g1) > 0 && ppt > 2) ? 2 : ppt;
 }
 
-// LDS floats of the dL/ddata box of the fused 2D backward
-constexpr int BOX_FLOATS = 4096;
+// LDS words for the dL/ddata bins and point slots of the fused 2D backward
+// (a 512-point tile with 3 channels needs 3584 + footprint cells + 1)
+constexpr int BOX_FLOATS = 6144;
 
 size_t lds_tables(int ndim, int nC) {
     switch (ndim) {
