@@ -265,28 +265,71 @@ CPAB_HD __forceinline__ void series_eval(const float (&q0)[D], const float (&W)[
     }
 }
 
-// y = exp(s M) u by nested Horner: y = u + s M (u + s/2 M (u + ...)).
+// E = exp(N), N = s M, by the Taylor series to N^KS with every power reduced
+// by Cayley-Hamilton to the span of {I, N} (2D: N^2 = tr N - det I) or
+// {I, N, N^2} (3D: N^3 = tr N^2 - sigma N + det I): scalar recurrences for the
+// coordinates of N^k, then one combination.
 template <int D>
-CPAB_HD __forceinline__ void expm_apply(const float (&M)[D][D], float s, const float (&u)[D], float (&y)[D]) {
-    float r[D];
+CPAB_HD __forceinline__ void exp_mat(const float (&M)[D][D], float s, float (&E)[D][D]) {
+    float N[D][D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) r[a] = u[a];
+    for (int r = 0; r < D; ++r)
 #pragma unroll
-    for (int k = KS; k >= 1; --k) {
-        const float f = s * (1.0f / (float)k);
-        float t[D];
+        for (int c = 0; c < D; ++c) N[r][c] = s * M[r][c];
+    if constexpr (D == 2) {
+        const float tr = N[0][0] + N[1][1];
+        const float det = fmaf(N[0][0], N[1][1], -N[0][1] * N[1][0]);
+        // N^k = a N + b I;  exp(N) = al N + be I
+        float a = 1.f, b = 0.f, al = 1.f, be = 1.f, fac = 1.f;
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-            float acc = 0.f;
-#pragma unroll
-            for (int a = 0; a < D; ++a) acc = fmaf(M[i][a], r[a], acc);
-            t[i] = fmaf(f, acc, u[i]);
+        for (int k = 2; k <= KS; ++k) {
+            const float an = fmaf(tr, a, b);
+            b = -det * a;
+            a = an;
+            fac *= 1.0f / (float)k;
+            al = fmaf(a, fac, al);
+            be = fmaf(b, fac, be);
         }
 #pragma unroll
-        for (int a = 0; a < D; ++a) r[a] = t[a];
-    }
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int a = 0; a < D; ++a) y[a] = r[a];
+            for (int c = 0; c < 2; ++c) E[r][c] = fmaf(al, N[r][c], (r == c) ? be : 0.f);
+    } else {
+        float N2[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc = fmaf(N[r][k], N[k][c], acc);
+                N2[r][c] = acc;
+            }
+        const float tr = N[0][0] + N[1][1] + N[2][2];
+        const float sg = fmaf(N[0][0], N[1][1], -N[0][1] * N[1][0]) + fmaf(N[0][0], N[2][2], -N[0][2] * N[2][0]) +
+                         fmaf(N[1][1], N[2][2], -N[1][2] * N[2][1]);
+        const float det = N[0][0] * fmaf(N[1][1], N[2][2], -N[1][2] * N[2][1]) -
+                          N[0][1] * fmaf(N[1][0], N[2][2], -N[1][2] * N[2][0]) +
+                          N[0][2] * fmaf(N[1][0], N[2][1], -N[1][1] * N[2][0]);
+        // N^k = a N^2 + b N + c I, from k = 2;  exp(N) = ga N^2 + be N + al I
+        float a = 1.f, b = 0.f, c = 0.f, ga = 0.5f, be = 1.f, al = 1.f, fac = 0.5f;
+#pragma unroll
+        for (int k = 3; k <= KS; ++k) {
+            const float an = fmaf(tr, a, b);
+            const float bn = fmaf(-sg, a, c);
+            c = det * a;
+            a = an;
+            b = bn;
+            fac *= 1.0f / (float)k;
+            ga = fmaf(a, fac, ga);
+            be = fmaf(b, fac, be);
+            al = fmaf(c, fac, al);
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) E[r][cc] = fmaf(ga, N2[r][cc], fmaf(be, N[r][cc], (r == cc) ? al : 0.f));
+    }
 }
 
 // Series coefficients of G(s) = n.q(s) + e for facet f of cell c.
@@ -754,15 +797,21 @@ CPAB_HD __forceinline__ void step_point(PointState<D>& st, const CellTable<D>& t
         Chunk<D> ch;
         advance<D>(st, tab, tol0, ch);
         if constexpr (WITHJ) {
+            float E[D][D], Jn[D][D];
+            exp_mat<D>(ch.M, ch.h, E);
 #pragma unroll
-            for (int j = 0; j < D; ++j) {
-                float u[D], y[D];
+            for (int r = 0; r < D; ++r)
 #pragma unroll
-                for (int a = 0; a < D; ++a) u[a] = J[a][j];
-                expm_apply<D>(ch.M, ch.h, u, y);
+                for (int j = 0; j < D; ++j) {
+                    float acc = 0.f;
 #pragma unroll
-                for (int a = 0; a < D; ++a) J[a][j] = y[a];
-            }
+                    for (int a = 0; a < D; ++a) acc = fmaf(E[r][a], J[a][j], acc);
+                    Jn[r][j] = acc;
+                }
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int j = 0; j < D; ++j) J[r][j] = Jn[r][j];
             if (ch.facet >= 0) saltation_fwd<D>(tab, ch, J);
         }
     }
