@@ -11,8 +11,8 @@
 // an entire series.  It is evaluated on chunks with |M|_F h <= RHO = 1/2 and
 // KS = 7 terms, so the truncation (< RHO^7/8! = 1.9e-7 relative to the chunk
 // displacement; 1e-9 at the typical |M|_F h ~ 0.25) is at FP32 resolution; no
-// eigenvalue case analysis
-// (real / complex / repeated) and no transcendental calls are needed in 2D/3D.
+// eigenvalue case analysis (real / complex / repeated) and no transcendental
+// calls are needed in 2D/3D.
 // 1D uses the closed form q(s) = q0 + s*phi1(a s)*v0 and the analytic hit time
 // s = (qb-q0)/v0 * log1p(r)/r, r = a (qb-q0)/v0.
 //
