@@ -126,19 +126,32 @@ CPAB_HD __forceinline__ float horner1(const float (&cf)[KS + 1], float s) {
 }
 
 // Root of G in [lo, hi] with G(lo) <= 0 < G(hi): Newton steps from the
-// regula-falsi point, safeguarded by bisection (fixed count: lanes of a wave
-// solve together).
+// regula-falsi point, safeguarded by bisection.  A lane stops after a Newton
+// step below 1e-5 of its bracket (quadratic convergence: the error is then
+// ~1e-10) or after 6 steps; its result depends on its own iterates only, not
+// on the other lanes of the wave.
 CPAB_HD __forceinline__ float solve_bracket(const float (&cf)[KS + 1], float lo, float hi, float glo, float ghi) {
     float s = lo + (hi - lo) * (glo * fast_rcp(glo - ghi));
     if (!(s >= lo && s <= hi)) s = 0.5f * (lo + hi);
+    const float tol = 1e-5f * (hi - lo);
+    bool fin = false;
 #pragma unroll
     for (int it = 0; it < 6; ++it) {
-        float g, d;
-        horner2(cf, s, g, d);
-        if (g > 0.f) hi = s; else lo = s;
-        float sn = s - g * fast_rcp(d);
-        if (!(sn >= lo && sn <= hi)) sn = 0.5f * (lo + hi);
-        s = sn;
+        if (!fin) {
+            float g, d;
+            horner2(cf, s, g, d);
+            if (g > 0.f) hi = s; else lo = s;
+            float sn = s - g * fast_rcp(d);
+            const bool newton = sn >= lo && sn <= hi;
+            if (!newton) sn = 0.5f * (lo + hi);
+            fin = newton && fabsf(sn - s) <= tol;
+            s = sn;
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (__ballot(!fin) == 0) break;
+#else
+        if (fin) break;
+#endif
     }
     return s;
 }
