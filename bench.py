@@ -48,7 +48,7 @@ def run_steps(T, theta, data, target, size, steps):
         th.grad = None
         dd.grad = None
         out = T.transform_data(dd, th, size)
-        loss = ((out - target) ** 2).mean()
+        loss = torch.nn.functional.mse_loss(out, target)
         loss.backward()
     return th, dd
 
@@ -263,7 +263,7 @@ def run_config(args):
         ths = [t.detach().clone().requires_grad_(True) for t in thetas]
         dd = data.detach().clone().requires_grad_(data_grad)
         out = S.transform_data(dd, ths, size) if seq else Ts[0].transform_data(dd, ths[0], size)
-        loss = ((out - target) ** 2).mean()
+        loss = torch.nn.functional.mse_loss(out, target)
         loss.backward()
         return ths
 

@@ -6,6 +6,7 @@ These replace the reference's _CPABFunction_AnalyticGrad
 stream, buffers come from torch's caching allocator, and nothing synchronises
 the host.
 """
+import collections
 import ctypes
 
 import numpy as np
@@ -187,9 +188,22 @@ class Interpolate(torch.autograd.Function):
 # --------------------------------------------------------------------------
 # fused transform_data
 # --------------------------------------------------------------------------
+_AXES = collections.OrderedDict()
+
+
 def meshgrid_axes(outsize, device):
-    """The axis coordinates of libcpab/pytorch/functions.py:102-108, concatenated."""
-    return torch.cat([torch.linspace(0, 1, int(n), device=device) for n in outsize]).contiguous()
+    """The axis coordinates of libcpab/pytorch/functions.py:102-108, concatenated.
+    Built once per (outsize, device) and kept (read-only) for later calls."""
+    key = (tuple(int(n) for n in outsize), device.type, device.index)
+    lin = _AXES.get(key)
+    if lin is None:
+        lin = torch.cat([torch.linspace(0, 1, int(n), device=device) for n in outsize]).contiguous()
+        if lin.is_cuda:
+            torch.cuda.current_stream(device).synchronize()  # usable from any stream from now on
+        _AXES[key] = lin
+        if len(_AXES) > 64:
+            _AXES.popitem(last=False)
+    return lin
 
 
 class TransformData(torch.autograd.Function):
