@@ -599,10 +599,10 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     int bx0 = 0, by0 = 0, bh = 1, cells = 0;
     const int nslot = L.ppt * BLOCK;
     int* bins = reinterpret_cast<int*>(box);  // [cells + 1] counts, then offsets
-    int* code = bins;                         // [nslot] cell | rank << 11 | flags
+    int* code = bins;                         // [nslot] cell | rank << 11
     int* list = bins;                         // [nslot] slots by bin
-    float* fdv = box;                         // [2, nslot]
-    float* gvv = box;                         // [C, nslot]
+    float2* fd2 = nullptr;                    // [nslot] corner weights (0 where a corner is clamped)
+    float4* gv4 = nullptr;                    // [nslot] grad_out of the channels
     if constexpr (D == 2 && HAVES) {
         if (box && gdata) {  // uniform over the block
             if (threadIdx.x == 0) {
@@ -640,14 +640,14 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             by0 = s_bb[2];
             bh = s_bb[3] - by0 + 1;
             cells = (s_bb[1] - bx0 + 1) * bh;
-            use_box = bx0 != INT_MAX && im.C <= 4 && cells < BIN_CELLS &&
-                      (long long)cells + 1 + (long long)nslot * (4 + im.C) <= box_cap;
+            const long long slot0 = ((long long)cells + 1 + 2LL * nslot + 3) & ~3LL;  // 16-byte aligned
+            use_box = bx0 != INT_MAX && im.C <= 4 && cells < BIN_CELLS && slot0 + 6LL * nslot <= box_cap;
             __syncthreads();  // s_bb is reused by the next tile of a persistent block
             if (use_box) {
                 code = bins + cells + 1;
                 list = code + nslot;
-                fdv = reinterpret_cast<float*>(list + nslot);
-                gvv = fdv + 2 * nslot;
+                fd2 = reinterpret_cast<float2*>(box + slot0);
+                gv4 = reinterpret_cast<float4*>(fd2 + nslot);
                 for (int i = threadIdx.x; i <= cells; i += BLOCK) bins[i] = 0;
                 __syncthreads();
             }
@@ -694,20 +694,22 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             float fd[3], dfd[3] = {0.f, 0.f, 0.f};
             corners<D>(y, im, i0, i1, fd);
             const int slot = r * BLOCK + threadIdx.x;
+            float gk[4] = {0.f, 0.f, 0.f, 0.f};
             for (int c = 0; c < im.C; ++c) {
                 const size_t base = (size_t)c * dstride;
                 const float gv = gout[(size_t)c * L.nP + t];
                 interp_backward<D>(data + base, (gdata && !use_box) ? gdata + base : nullptr, im, i0, i1, fd, gv,
                                    dfd);
-                if (use_box) gvv[c * nslot + slot] = gv;
+                if (c < 4) gk[c] = gv;
             }
             if constexpr (D == 2) {
                 if (use_box) {
                     const int k = (i0[0] - bx0) * bh + (i0[1] - by0);
                     const int rank = atomicAdd(&bins[k], 1);
-                    code[slot] = k | (rank << 11) | ((i1[0] != i0[0]) << 29) | ((i1[1] != i0[1]) << 30);
-                    fdv[slot] = fd[0];
-                    fdv[nslot + slot] = fd[1];
+                    code[slot] = k | (rank << 11);
+                    // a clamped corner pair (i1 == i0) puts its whole weight on i0
+                    fd2[slot] = make_float2(i1[0] != i0[0] ? fd[0] : 0.f, i1[1] != i0[1] ? fd[1] : 0.f);
+                    gv4[slot] = make_float4(gk[0], gk[1], gk[2], gk[3]);
                 }
             }
             float w[D];
@@ -734,22 +736,22 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
         for (int k = threadIdx.x; k < cells; k += BLOCK) {
             const int i = k / bh, j = k - i * bh;
             float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
             for (int a = 0; a < 2; ++a)
+#pragma unroll
                 for (int b = 0; b < 2; ++b) {
                     if (i < a || j < b) continue;
                     const int kb = k - a * bh - b;
-                    for (int e = bins[kb]; e < bins[kb + 1]; ++e) {
+                    const int e1 = bins[kb + 1];
+                    for (int e = bins[kb]; e < e1; ++e) {
                         const int sl = list[e];
-                        const int cd = code[sl];
-                        const bool rf = (cd >> 29) & 1, cf = (cd >> 30) & 1;
-                        if ((a && !rf) || (b && !cf)) continue;
-                        const float f0 = fdv[sl], f1 = fdv[nslot + sl];
-                        const float wr = a ? f0 : (rf ? 1.f - f0 : 1.f);
-                        const float wc = b ? f1 : (cf ? 1.f - f1 : 1.f);
-                        const float w = wr * wc;
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            if (c < im.C) acc[c] = fmaf(w, gvv[c * nslot + sl], acc[c]);
+                        const float2 f = fd2[sl];
+                        const float4 gq = gv4[sl];
+                        const float w = (a ? f.x : 1.f - f.x) * (b ? f.y : 1.f - f.y);
+                        acc[0] = fmaf(w, gq.x, acc[0]);
+                        acc[1] = fmaf(w, gq.y, acc[1]);
+                        acc[2] = fmaf(w, gq.z, acc[2]);
+                        acc[3] = fmaf(w, gq.w, acc[3]);
                     }
                 }
 #pragma unroll
@@ -1303,7 +1305,8 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
 }
 
 // LDS words for the dL/ddata bins and point slots of the fused 2D backward
-// (a 512-point tile with 3 channels needs 3584 + footprint cells + 1)
+// (a 512-point tile needs 4096 + footprint cells + 1 words: up to ~2040
+// cells, a 16 x 32 tile at up to ~2x magnification)
 constexpr int BOX_FLOATS = 6144;
 
 size_t lds_tables(int ndim, int nC) {
