@@ -834,34 +834,37 @@ __global__ __launch_bounds__(BLOCK) void k_interp_bwd(int N, int nP, Src s, Img 
 }
 
 // per theta: dL/dA (x-space) = sum over tiles of the q-space partials, scaled
-// back; dL/dtheta = B^T dL/dA.  One block per theta, fixed summation order.
-__global__ __launch_bounds__(BLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
-                                                        int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
-                                                        float* __restrict__ gtheta, float* __restrict__ gA) {
-    extern __shared__ float sg[];  // [WAVES + 1, Dt]
+// back; dL/dtheta = B^T dL/dA.  One block of nw = blockDim.x / 64 waves per
+// theta (memory-bound: many waves keep loads in flight); every sum runs in a
+// fixed order.
+constexpr int RBLOCK = 1024;
+__global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
+                                                         int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
+                                                         float* __restrict__ gtheta, float* __restrict__ gA) {
+    extern __shared__ float sg[];  // [nw + 1, Dt]
     const int th = blockIdx.x;
+    const int nw = blockDim.x >> 6;
     const int P = ndim * (ndim + 1);
     const int nc[3] = {nc0, nc1, nc2};
     const float* pp = part + (size_t)th * tiles * Dt;
-    // wave w sums tiles w, w + WAVES, ... (two chains for overlap); the wave
-    // sums are then added in wave order
-    const int w = threadIdx.x >> 6;
-    for (int i = threadIdx.x & 63; i < Dt; i += 64) {
+    // wave w sums tiles w, w + nw, ... (two chains for overlap); the wave sums
+    // are then added in wave order
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = lane; i < Dt; i += 64) {
         float s0 = 0.f, s1 = 0.f;
         int t = w;
-        for (; t + WAVES < tiles; t += 2 * WAVES) {
+        for (; t + nw < tiles; t += 2 * nw) {
             s0 += pp[(size_t)t * Dt + i];
-            s1 += pp[(size_t)(t + WAVES) * Dt + i];
+            s1 += pp[(size_t)(t + nw) * Dt + i];
         }
         if (t < tiles) s0 += pp[(size_t)t * Dt + i];
         sg[w * Dt + i] = s0 + s1;
     }
     __syncthreads();
-    float* sA = sg + WAVES * Dt;
-    for (int i = threadIdx.x; i < Dt; i += BLOCK) {
+    float* sA = sg + nw * Dt;
+    for (int i = threadIdx.x; i < Dt; i += blockDim.x) {
         float s = sg[i];
-#pragma unroll
-        for (int k = 1; k < WAVES; ++k) s += sg[k * Dt + i];
+        for (int k = 1; k < nw; ++k) s += sg[k * Dt + i];
         const int rem = i % P, r = rem / (ndim + 1), col = rem % (ndim + 1);
         const float sc = (col < ndim) ? (float)nc[r] / (float)nc[col] : (float)nc[r];
         s *= sc;
@@ -869,9 +872,18 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_theta(const float* __restrict_
         if (gA) gA[(size_t)th * Dt + i] = s;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < d; k += BLOCK) {
+    // B^T dL/dA: wave w takes rows [i0, i1) of B, the wave partials (reusing
+    // sg[0, nw*d), d <= Dt) are added in wave order
+    const int i0 = (int)((long long)Dt * w / nw), i1 = (int)((long long)Dt * (w + 1) / nw);
+    for (int k = lane; k < d; k += 64) {
         float s = 0.f;
-        for (int i = 0; i < Dt; ++i) s = fmaf(B[(size_t)i * d + k], sA[i], s);
+        for (int i = i0; i < i1; ++i) s = fmaf(B[(size_t)i * d + k], sA[i], s);
+        sg[w * d + k] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < d; k += blockDim.x) {
+        float s = sg[k];
+        for (int w2 = 1; w2 < nw; ++w2) s += sg[w2 * d + k];
         gtheta[(size_t)th * d + k] = s;
     }
 }
@@ -1342,8 +1354,11 @@ int launch_reduce(const cpab_tess_t* t, const float* part, int n_theta, int tile
                   hipStream_t st) {
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
     Geo g = make_geo(t);
-    if ((size_t)(WAVES + 1) * Dt * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
-    hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(BLOCK), (WAVES + 1) * Dt * sizeof(float), st, part, tiles,
+    // as many waves (up to 16) as the LDS holds [nw + 1, Dt] floats for
+    int nw = RBLOCK / 64;
+    while (nw > 1 && (size_t)(nw + 1) * Dt * sizeof(float) > LDS_LIMIT) nw >>= 1;
+    if ((size_t)(nw + 1) * Dt * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
+    hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(nw * 64), (nw + 1) * Dt * sizeof(float), st, part, tiles,
                        Dt, t->ndim, g.nc0, g.nc1, g.nc2, t->B, t->d, gtheta, gA);
     return status_from_hip();
 }
