@@ -190,12 +190,16 @@ struct Bracket {
 
 // Isolates the first time in [0, h] at which the trajectory leaves through the
 // facet with coefficients cf (G(s) = sum_k cf[k] s^k, G <= 0 inside).
-//  * A start on the facet (G(0) >= -tol0) moving out exits at 0 when
-//    allow_start.
-//  * A start on the facet moving in first dips below 0; a later positive
-//    sample is bracketed from the dip's minimum (zero of G').
-//  * Any other start on the facet (start exit disallowed, or tangential) is
-//    ignored until G has gone negative.
+//  * A start on the facet (G(0) >= -tol0, taken as G(0) = 0) moving out
+//    exits at 0 when allow_start.
+//  * A start on the facet with G'(0) < 0, however small, first dips below 0;
+//    a later positive sample is bracketed from the dip's minimum (zero of
+//    G').  Two cells round the normal speed of the same point differently, so
+//    a point handed over by a zero-length crossing may see a tangential start
+//    here that its previous cell saw as a clear exit: the dip still has to be
+//    followed out.
+//  * Any other start on the facet (G'(0) >= 0 with the start exit
+//    disallowed) is ignored until G has gone negative.
 //  * Between two negative samples a local maximum above 0 is a crossing.
 CPAB_HD __forceinline__ void first_bracket(const float (&cf)[KS + 1], float h, float tol0, float tol1, bool allow_start,
                                            Bracket& br) {
@@ -206,7 +210,7 @@ CPAB_HD __forceinline__ void first_bracket(const float (&cf)[KS + 1], float h, f
         br.lo = br.hi = 0.f;
         return;
     }
-    bool skip = on && !(cf[1] < -tol1);
+    bool skip = on && !(cf[1] < 0.f);
     bool dip = on && !skip;
     float ps = 0.f, pg = cf[0], pd = cf[1];
 #pragma unroll 1
@@ -346,16 +350,19 @@ CPAB_HD __forceinline__ void exp_mat(const float (&M)[D][D], float s, float (&E)
     }
 }
 
-// Series coefficients of G(s) = n.q(s) + e for facet f of cell c.
+// Series coefficients of G(s) = n.q(s) + e for facet f of cell c.  A start
+// within tol0 of the facet is a start on it: G(0) = 0 exactly, so the sign of
+// G just after the start is that of G'(0) (and, for a shallow dip into the
+// cell, of the dip) rather than that of the rounding in n.q + e.
 template <int D>
 CPAB_HD __forceinline__ void facet_coeffs(const CellTable<D>& tab, int c, int f, const float (&q)[D],
-                                          const float (&W)[KS][D], float (&cf)[KS + 1]) {
+                                          const float (&W)[KS][D], float tol0, float (&cf)[KS + 1]) {
     const float4 P = tab.pl[c * NF + f];
     const float n[3] = {P.x, P.y, P.z};
     float g0 = P.w;
 #pragma unroll
     for (int a = 0; a < D; ++a) g0 = fmaf(n[a], q[a], g0);
-    cf[0] = g0;
+    cf[0] = (g0 >= -tol0) ? 0.f : g0;
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
         float acc = 0.f;
@@ -480,7 +487,7 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
             const int f = __builtin_ctz(slow);
             slow &= slow - 1u;
             float cf[KS + 1];
-            facet_coeffs<D>(tab, c, f, st.q, ch.W, cf);
+            facet_coeffs<D>(tab, c, f, st.q, ch.W, tol0, cf);
             const int nbf = tab.nb[c * NF + f];
             const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
             Bracket br;
@@ -521,7 +528,7 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
             }
         }
         float cf[KS + 1];
-        facet_coeffs<D>(tab, c, bf, st.q, ch.W, cf);
+        facet_coeffs<D>(tab, c, bf, st.q, ch.W, tol0, cf);
         best = solve_bracket(cf, lo, hi, glo, ghi);
         unsigned others = cand & ~(1u << bf);
 #pragma unroll 1
@@ -537,7 +544,7 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
                     ghi = cghi[k];
                 }
             if (!(lo < best)) continue;
-            facet_coeffs<D>(tab, c, f, st.q, ch.W, cf);
+            facet_coeffs<D>(tab, c, f, st.q, ch.W, tol0, cf);
             if (best < hi) {
                 hi = best;
                 ghi = horner1(cf, best);

@@ -104,3 +104,56 @@ def test_host_build_jacobian_matches_finite_differences(nc, zb, scale):
     err = np.abs(Jx - fd).max(axis=(1, 2)) / np.maximum(1.0, np.abs(fd).max(axis=(1, 2)))
     # points whose trajectory passes within eps of a vertex see a kink in the differences
     assert np.quantile(err, 0.98) <= 1e-3, np.quantile(err, 0.98)
+
+
+def _facet_lines_2d(nx, ny):
+    """Interior facets of the 2D tessellation as q-space segments (p0, p1) with
+    unit normals: both diagonals of every cell square, and the interior edges
+    between squares."""
+    lines = []
+    r2 = 1.0 / np.sqrt(2.0)
+    for i in range(nx):
+        for j in range(ny):
+            lines.append(((i, j), (i + 1, j + 1), (r2, -r2)))
+            lines.append(((i, j + 1), (i + 1, j), (r2, r2)))
+    for i in range(1, nx):
+        for j in range(ny):
+            lines.append(((i, j), (i, j + 1), (1.0, 0.0)))
+    for j in range(1, ny):
+        for i in range(nx):
+            lines.append(((i, j), (i + 1, j), (0.0, 1.0)))
+    return lines
+
+
+@pytest.mark.parametrize("nc,zb,scale", [([4, 4], True, 0.3), ([3, 3], False, 0.5), ([5, 3], True, 0.4)])
+def test_host_starts_on_facets_with_shallow_dips(nc, zb, scale):
+    """Starts on a facet near the point where the normal velocity vanishes:
+    the trajectory dips into the cell by ~G'(0)^2 / (4 G''(0)/2), far less
+    than the rounding of G(0) in FP32, and leaves again through the same
+    facet (or grazes it).  The crossing back must be found, or the point goes
+    on with the wrong cell's field for the rest of the time."""
+    t = make_tessellation(nc, zero_boundary=zb)
+    planes, nbr = t.facet_table()
+    rng = np.random.default_rng(11 + sum(nc))
+    theta = rng.standard_normal((3, t.B.shape[1])) * scale
+    A = O.expand_A(t.B, theta, 2)
+    S = np.array(nc, dtype=np.float64)[:, None]
+    offsets = np.array([0.0, 1e-6, -1e-6, 1e-5, -1e-5, 1e-4, -1e-4, 1e-3, -1e-3])
+    for k in range(theta.shape[0]):
+        pts = []
+        for p0, p1, n in _facet_lines_2d(*nc):
+            p0, p1, n = np.array(p0, float), np.array(p1, float), np.array(n)
+            u = np.array([0.25, 0.75])
+            q = p0[:, None] + u[None] * (p1 - p0)[:, None]
+            g = n @ (O.vectorfield(2, nc, A[k], q / S) * S)  # normal velocity, linear along the facet
+            if g[0] == g[1]:
+                continue
+            ustar = 0.25 - g[0] * 0.5 / (g[1] - g[0])
+            uu = ustar + offsets
+            uu = uu[(uu > 0.02) & (uu < 0.98)]
+            pts.append((p0[:, None] + uu[None] * (p1 - p0)[:, None]) / S)
+        grid = np.concatenate(pts, 1).astype(np.float32)
+        got = host_transform(nc, t.B, planes, nbr, theta[k:k + 1], grid)
+        ref = O.cf_forward(2, nc, A[k:k + 1], planes, nbr, grid.astype(np.float64))
+        err = np.abs(got - ref).max()
+        assert err <= 1e-5, (k, err, grid[:, np.abs(got - ref).max(1).argmax()])
