@@ -6,9 +6,12 @@
  * outputs, gradients and the workspace (size from cpab_workspace_bytes,
  * cpab_data_workspace_bytes or cpab_batch_workspace_bytes) are
  * owned by the caller, and nothing is zero-filled inside the library except
- * where stated (grad_data must arrive zero-filled).  Every launch goes to the
- * given stream; no call synchronises the host.  Return value: CPAB_OK (0) or a
- * CPAB_ERR_* code, text from cpab_status_string().
+ * where stated (grad_data must arrive zero-filled).  A pointer to an array
+ * with no elements (nP, N or n_theta of 0) may be NULL; such a call does
+ * nothing except write the zeros of a gradient whose shape is not empty.
+ * Every launch goes to the given stream; no call synchronises the host.
+ * Return value: CPAB_OK (0) or a CPAB_ERR_* code, text from
+ * cpab_status_string().
  *
  * Layouts follow the reference's pytorch backend:
  *   theta  [n_theta, d] f32; B [D, d] f32 with D = nC * ndim * (ndim+1)

@@ -1414,8 +1414,9 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
                        hipStream_t stream) {
     int rc = check_tess(t);
     if (rc) return rc;
-    if (!theta || !points || !out || n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
-    if (n_theta == 0 || nP == 0) return CPAB_OK;
+    if (n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0 || nP == 0) return CPAB_OK;  // arrays with no elements may be NULL
+    if (!theta || !points || !out) return CPAB_ERR_ARG;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
     const size_t lds = lds_tables(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
@@ -1448,8 +1449,10 @@ int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, co
                        float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream) {
     int rc = check_tess(t);
     if (rc) return rc;
-    if (!theta || !points || !grad_out || !grad_theta || n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
+    if (n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
     if (n_theta == 0) return CPAB_OK;
+    // no points: grad_theta is written with zeros; points and grad_out may be NULL
+    if (!theta || !grad_theta || (nP > 0 && (!points || !grad_out))) return CPAB_ERR_ARG;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
     const size_t lds = lds_bwd(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
@@ -1512,8 +1515,9 @@ int cpab_interpolate_fwd(int ndim, const float* data, int N, int C, const int* d
     int nP, rc;
     if ((rc = img_from(ndim, data, C, dsize, im))) return rc;
     if ((rc = src_out(ndim, outsize, s, nP))) return rc;
-    if (!grid || !out || N < 0) return CPAB_ERR_ARG;
+    if (N < 0) return CPAB_ERR_ARG;
     if (N == 0) return CPAB_OK;
+    if (!grid || !out) return CPAB_ERR_ARG;
     s.pts = grid;
     const size_t total = (size_t)N * nP;
     dim3 g((unsigned)((total + BLOCK - 1) / BLOCK));
@@ -1532,8 +1536,9 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
     int nP, rc;
     if ((rc = img_from(ndim, data, C, dsize, im))) return rc;
     if ((rc = src_out(ndim, outsize, s, nP))) return rc;
-    if (!grid || !grad_out || N < 0) return CPAB_ERR_ARG;
+    if (N < 0) return CPAB_ERR_ARG;
     if (N == 0) return CPAB_OK;
+    if (!grid || !grad_out) return CPAB_ERR_ARG;
     im.gdata = grad_data;
     s.pts = grid;
     const size_t total = (size_t)N * nP;
@@ -1644,8 +1649,9 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
 
 int cpab_findcell(int ndim, const int* nc, const float* points, int nP, int* idx, hipStream_t stream) {
     if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
-    if (!nc || !points || !idx || nP < 0) return CPAB_ERR_ARG;
+    if (!nc || nP < 0) return CPAB_ERR_ARG;
     if (nP == 0) return CPAB_OK;
+    if (!points || !idx) return CPAB_ERR_ARG;
     const int n0 = nc[0], n1 = ndim > 1 ? nc[1] : 1, n2 = ndim > 2 ? nc[2] : 1;
     dim3 g((nP + BLOCK - 1) / BLOCK);
 #define LAUNCH_FC(DD) hipLaunchKernelGGL((k_findcell<DD>), g, dim3(BLOCK), 0, stream, n0, n1, n2, points, nP, idx);
@@ -1658,8 +1664,9 @@ int cpab_vectorfield(const cpab_tess_t* t, const float* theta, const float* poin
                      size_t ws_bytes, hipStream_t stream) {
     int rc = check_tess(t);
     if (rc) return rc;
-    if (!theta || !points || !v || nP < 0) return CPAB_ERR_ARG;
+    if (nP < 0) return CPAB_ERR_ARG;
     if (nP == 0) return CPAB_OK;
+    if (!theta || !points || !v) return CPAB_ERR_ARG;
     if (ws_bytes < cpab_workspace_bytes(t, 1, nP) || !workspace) return CPAB_ERR_WORKSPACE;
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, 1, Aq, stream))) return rc;

@@ -68,6 +68,12 @@ def test_argument_errors_are_status_codes():
     assert L.cpab_transform_fwd(ctypes.byref(t), 1, 4, 1, 1000, 0, 1.0, 1, None, 1, need - 1, None) == 3
     # empty problems are a no-op
     assert L.cpab_transform_fwd(ctypes.byref(t), 1, 0, 1, 1000, 0, 1.0, 1, None, None, 0, None) == 0
+    # ... and their empty arrays may be NULL
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 2, None, 0, 0, 1.0, None, None, None, 0, None) == 0
+    assert L.cpab_transform_fwd(ctypes.byref(t), None, 0, None, 0, 0, 1.0, None, None, None, 0, None) == 0
+    assert L.cpab_findcell(2, (ctypes.c_int * 2)(3, 3), None, 0, None, None) == 0
+    # a negative size is an error before any pointer is looked at
+    assert L.cpab_transform_fwd(ctypes.byref(t), None, 2, None, -1, 0, 1.0, None, None, None, 0, None) == 1
     assert L.cpab_interpolate_fwd(5, 1, 1, 1, None, 1, None, 1, None) == 2
     dsz = (ctypes.c_int * 3)(4, 4, 1)
     osz = (ctypes.c_int * 3)(0, 4, 1)
