@@ -191,7 +191,8 @@ struct Bracket {
 // Isolates the first time in [0, h] at which the trajectory leaves through the
 // facet with coefficients cf (G(s) = sum_k cf[k] s^k, G <= 0 inside).
 //  * A start on the facet (G(0) >= -tol0, taken as G(0) = 0) moving out
-//    exits at 0 when allow_start.
+//    exits at 0 when allow_start: G'(0) > tol1, or 0 <= G'(0) <= tol1 (an
+//    outward speed within the rounding of the field) with G''(0) > 0.
 //  * A start on the facet with G'(0) < 0, however small, first dips below 0;
 //    a later positive sample is bracketed from the dip's minimum (zero of
 //    G').  Two cells round the normal speed of the same point differently, so
@@ -205,7 +206,7 @@ CPAB_HD __forceinline__ void first_bracket(const float (&cf)[KS + 1], float h, f
                                            Bracket& br) {
     br.idx = NSAMP + 1;
     const bool on = cf[0] >= -tol0;
-    if (on && allow_start && (cf[1] > tol1 || (fabsf(cf[1]) <= tol1 && cf[2] > 0.f))) {
+    if (on && allow_start && (cf[1] > tol1 || (cf[1] >= 0.f && cf[2] > 0.f))) {
         br.idx = 0;
         br.lo = br.hi = 0.f;
         return;
@@ -401,7 +402,21 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
 #pragma unroll
     for (int a = 0; a < D; ++a) vn2 = fmaf(v0[a], v0[a], vn2);
     const float vn = sqrtf(vn2);
-    const float tol1 = 1e-6f * vn;
+    // Tangential band for an outward normal speed G'(0): the neighbouring
+    // cells' FP32 fields agree on a shared facet only up to the rounding of
+    // M q + b, whose terms can be much larger than |v0| where they cancel, so
+    // both may round a vanishing normal speed to a small exit.  Inside the
+    // band a start exit also needs G'' > 0 (the same side from both cells for
+    // a tangent velocity); an inward G'(0) always follows the dip.
+    float vabs = 0.f;
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        float t = fabsf(ch.b[r]);
+#pragma unroll
+        for (int a = 0; a < D; ++a) t = fmaf(fabsf(ch.M[r][a]), fabsf(st.q[a]), t);
+        vabs += t;
+    }
+    const float tol1 = 1e-6f * vabs;
     // Bounds on the chunk (|n| = 1, |M| <= |M|_F, |M|_F h <= RHO, |cf_k| <=
     // |M|^(k-1) |v0| / k!):  |G''(s)| <= K = |M|_F |v0| e^RHO and
     // |G''(s) - 2 cf2 - 6 cf3 s| <= K4 = |M|_F^3 |v0| h^2 e^RHO / 2.
@@ -442,7 +457,7 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
         for (int a = 0; a < D; ++a) g2 = fmaf(n[a], ch.W[1][a], g2);
         if (on) {
             const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
-            if (allow_start && (g1 > tol1 || (fabsf(g1) <= tol1 && g2 > 0.f))) {
+            if (allow_start && (g1 > tol1 || (g1 >= 0.f && g2 > 0.f))) {
                 if (start_f < 0) start_f = f;
             } else {
                 slow |= 1u << f;
@@ -574,6 +589,10 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
         st.zero_hops = (step <= ZERO_HOP) ? st.zero_hops + 1 : 0;
         st.came_from = c;
         st.c = ch.next;
+    } else if (step > ZERO_HOP) {
+        // time went by inside the cell: the guard against crossing straight
+        // back only covers a run of zero-length crossings
+        st.zero_hops = 0;
     }
     st.iters++;
 }

@@ -69,3 +69,34 @@ extern "C" int host_findcell(int ndim, const int* nc, const float* pts, int nP, 
     }
     return 0;
 }
+
+// Integration from a given state (2D, q-space): cell c, remaining time T and
+// the record of the last crossing (came_from, zero_hops) as the state machine
+// holds them after a crossing.  q is overwritten with the end point.
+extern "C" int host_integrate_state2(const int* nc, int nC, const float* Aq, const float* planes, const int* nbr,
+                                     float* q, int c, float T, int came_from, int zero_hops, float tol0) {
+    constexpr int P = 6;
+    float* inv = (float*)malloc(sizeof(float) * nC);
+    for (int k = 0; k < nC; ++k) {
+        float s = 0.f;
+        for (int r = 0; r < 2; ++r)
+            for (int j = 0; j < 2; ++j) s += Aq[k * P + r * 3 + j] * Aq[k * P + r * 3 + j];
+        inv[k] = s > 0.f ? 1.0f / sqrtf(s) : INFINITY;
+    }
+    CellTable<2> tab;
+    tab.A = Aq;
+    tab.inv_n = inv;
+    tab.pl = reinterpret_cast<const float4*>(planes);
+    tab.nb = nbr;
+    PointState<2> st;
+    const float q0[2] = {q[0], q[1]};
+    init_point<2>(st, q0, c, T);
+    st.came_from = came_from;
+    st.zero_hops = zero_hops;
+    float J[2][2];
+    while (point_active<2>(st)) step_point<2, false>(st, tab, nc[0], tol0, J);
+    q[0] = st.q[0];
+    q[1] = st.q[1];
+    free(inv);
+    return 0;
+}

@@ -20,7 +20,8 @@ def _theta(T, n, scale, seed):
 
 
 @pytest.mark.parametrize("nc,n_theta,size,scale", [([4, 4], 256, [256, 256], 0.3), ([50], 4096, [1024], 0.5),
-                                                   ([3, 3, 3], 16, [64, 64, 64], 0.2)])
+                                                   ([3, 3, 3], 16, [64, 64, 64], 0.2), ([4, 4], 256, [256, 256], 1.0),
+                                                   ([3, 3, 3], 16, [64, 64, 64], 0.6), ([50], 4096, [1024], 1.0)])
 def test_inverse_round_trip_at_benchmark_size(cuda, nc, n_theta, size, scale):
     T = Cpab(nc, backend="pytorch", device="gpu", zero_boundary=True)
     theta = _theta(T, n_theta, scale, 0).to(cuda)
@@ -28,7 +29,9 @@ def test_inverse_round_trip_at_benchmark_size(cuda, nc, n_theta, size, scale):
     fwd = T.transform_grid(grid, theta)
     back = T.transform_grid(fwd, -theta)
     err = (back - grid[None]).abs().max().item()
-    assert err <= 2e-5, err
+    # FP32 rounding of both flows, amplified by the inverse's Jacobian (the
+    # FP64 flow of the FP32-rounded forward returns within 5e-8)
+    assert err <= 5e-6, err
     # a sample of the forward against the exact FP64 flow
     p = T.get_params()
     idx = torch.randperm(grid.shape[1], generator=torch.Generator().manual_seed(1))[:512]

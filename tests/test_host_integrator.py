@@ -157,3 +157,95 @@ def test_host_starts_on_facets_with_shallow_dips(nc, zb, scale):
         ref = O.cf_forward(2, nc, A[k:k + 1], planes, nbr, grid.astype(np.float64))
         err = np.abs(got - ref).max()
         assert err <= 1e-5, (k, err, grid[:, np.abs(got - ref).max(1).argmax()])
+
+
+def test_host_exit_through_the_facet_entered_by_a_zero_length_crossing():
+    """A point that entered its cell by a zero-length crossing, spent a whole
+    chunk (|M|_F h = 1/2) inside, and starts the next chunk within tol0 of the
+    facet it came in by while moving out: it must be allowed to leave through
+    that facet (the no-return guard covers only a run of zero-length
+    crossings).  The state is built by flowing back from a point on the facet
+    for one chunk plus a step that leaves it 2e-6 inside."""
+    nc = [4, 4]
+    t = make_tessellation(nc, zero_boundary=True)
+    planes, nbr = t.facet_table()
+    S = np.array(nc, dtype=np.float64)
+    tol0 = 2e-6 * max(nc)
+    lines = _facet_lines_2d(*nc)
+    rng = np.random.default_rng(21)
+    done = 0
+    for _ in range(400):
+        theta = rng.standard_normal((1, t.B.shape[1])) * 0.5
+        A = O.expand_A(t.B, theta, 2)
+        p0, p1, n = (np.array(v, float) for v in lines[rng.integers(len(lines))])
+        qr = p0 + rng.uniform(0.2, 0.8) * (p1 - p0)  # crossing point, q-space
+        v = O.vectorfield(2, nc, A[0], (qr / S)[:, None])[:, 0] * S
+        vn = float(n @ v)
+        if np.linalg.norm(v) < 0.05 or abs(vn) < 0.3 * np.linalg.norm(v):
+            continue
+        side = np.sign(vn) * n  # points into the cell the trajectory exits to
+        X = int(O.findcell(2, nc, ((qr + 1e-3 * side) / S)[:, None])[0])
+        Y = int(O.findcell(2, nc, ((qr - 1e-3 * side) / S)[:, None])[0])
+        Aq = A[0, Y, :, :2].astype(np.float32)
+        h = np.float32(0.5) / np.sqrt(np.float32((Aq * Aq).sum()))  # the kernel's chunk in cell Y
+        delta = 2e-6 / abs(vn)
+        back, seq, dur, nseg = O.cf_forward(2, nc, -A, planes, nbr, (qr / S)[:, None], T=float(h) + delta, max_segs=4)
+        if nseg[0, 0] != 1 or seq[0, 0, 0] != Y:
+            continue  # the backward path must stay inside Y
+        x0 = back[0, :, 0].astype(np.float32)
+        T = float(h) + delta + 0.25
+        ref = O.cf_forward(2, nc, A, planes, nbr, x0.astype(np.float64)[:, None], T=T)[0, :, 0]
+        Aqf = A[0] * np.concatenate([S[:, None] / S[None, :], S[:, None]], 1)  # q-space: nc_r/nc_c, b * nc_r
+        Aqf = np.ascontiguousarray(Aqf.reshape(-1, 6).astype(np.float32))
+        q = (x0 * np.array(nc, np.float32)).astype(np.float32)
+        ncarr = np.array(nc + [1], np.int32)
+        H_lib = _lib()
+        H_lib.host_integrate_state2(_ptr(ncarr), nbr.shape[0], _ptr(Aqf), _ptr(np.ascontiguousarray(planes, np.float32)),
+                                    _ptr(np.ascontiguousarray(nbr, np.int32)), _ptr(q), Y, ctypes.c_float(T), X, 1,
+                                    ctypes.c_float(tol0))
+        err = np.abs(q / S - ref).max()
+        assert err <= 1e-5, (done, err)
+        done += 1
+        if done == 20:
+            break
+    assert done == 20
+
+
+@pytest.mark.parametrize("nc,scale", [([3, 3, 3], 0.6), ([2, 2, 2], 0.8), ([3, 2, 2], 0.6)])
+def test_host_3d_starts_on_faces_near_tangency(nc, scale):
+    """Starts on the interior faces between cubes, moved along the face to
+    where the normal velocity vanishes (plus small offsets).  There the
+    normal speed is below the FP32 rounding of M q + b, and the two cells'
+    fields may both round it to a small exit; the side the point goes to must
+    come from the curvature instead."""
+    t = make_tessellation(nc, zero_boundary=True)
+    planes, nbr = t.facet_table()
+    rng = np.random.default_rng(3 + sum(nc))
+    theta = rng.standard_normal((2, t.B.shape[1])) * scale
+    A = O.expand_A(t.B, theta, 3)
+    S = np.array(nc, dtype=np.float64)[:, None]
+    for k in range(theta.shape[0]):
+        pts = []
+        for a in range(3):
+            b = (a + 1) % 3
+            for i in range(1, nc[a]):
+                q = rng.uniform(0.05, 0.95, size=(3, 300)) * S
+                q[a] = i
+                v = O.vectorfield(3, nc, A[k], q / S)[a] * nc[a]
+                q2 = q.copy()
+                q2[b] += 1e-3
+                dv = (O.vectorfield(3, nc, A[k], q2 / S)[a] * nc[a] - v) / 1e-3
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    shift = -v / dv
+                keep = np.isfinite(shift) & (np.abs(shift) < 0.05)
+                q = q[:, keep]
+                q[b] += shift[keep]
+                for off in (0.0, 1e-6, -1e-5, 1e-4):
+                    qq = q.copy()
+                    qq[b] += off
+                    pts.append(qq / S)
+        grid = np.clip(np.concatenate(pts, 1), 0.0, 1.0).astype(np.float32)
+        got = host_transform(nc, t.B, planes, nbr, theta[k:k + 1], grid)
+        ref = O.cf_forward(3, nc, A[k:k + 1], planes, nbr, grid.astype(np.float64))
+        err = np.abs(got - ref).max()
+        assert err <= 1e-5, (k, err, grid[:, np.abs(got - ref).max(1).argmax()])
