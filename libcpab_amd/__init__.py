@@ -5,6 +5,7 @@ hand-written HIP kernels on AMD Instinct MI355X (gfx950).
 The public API mirrors SkafteNicki/libcpab (libcpab/__init__.py:9-14).
 """
 from .cpab import Cpab
+from .alignment import CpabAligner
 from .sequential import CpabSequential
 
-__all__ = ["Cpab", "CpabSequential"]
+__all__ = ["Cpab", "CpabAligner", "CpabSequential"]
