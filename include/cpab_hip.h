@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CPAB_ABI_VERSION 2
+#define CPAB_ABI_VERSION 3
 
 #define CPAB_OK 0
 #define CPAB_ERR_ARG 1
@@ -66,7 +66,10 @@ size_t cpab_workspace_bytes(const cpab_tess_t* t, int n_theta, int nP);
  * _plain variants with n_theta = 1) onto a meshgrid of outsize[ndim]. */
 size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* outsize);
 
-/* Floats in the forward's saved state for the fused backward: n_theta*(ndim+ndim^2)*nP. */
+/* Floats in the forward's saved state for the fused backward:
+ * n_theta * (ndim + ndim^2 + (ndim > 1 ? 3 : 0)) * nP -- warped points,
+ * Jacobians and, in 2D/3D, the durations and crossed facets of each point's
+ * first two chunks, which the backward replays without searching again. */
 size_t cpab_saved_floats(int ndim, int n_theta, int nP);
 
 /* A_q = B theta per cell, in q-space.  Replaces the basis expansion of
@@ -107,7 +110,7 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
  * (Cpab.transform_data, libcpab/cpab.py:304-331).  lin holds the meshgrid axis
  * coordinates concatenated (outsize[0] + outsize[1] + ... floats; the values
  * of libcpab/pytorch/functions.py:102-108).  saved (nullable,
- * cpab_saved_floats) keeps the warped points and Jacobians for the backward. */
+ * cpab_saved_floats) keeps the warped points, Jacobians and chunk logs for the backward. */
 int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
                             const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
                             float* saved, void* workspace, size_t ws_bytes, hipStream_t stream);
@@ -161,7 +164,7 @@ typedef struct cpab_instance {
     const float* data;   /* [C, dsize...] */
     const float* lin;    /* outsize[0] + outsize[1] + outsize[2] meshgrid axis coordinates */
     float* out;          /* [C, outsize...] */
-    float* saved;        /* [ndim + ndim^2, nP] or NULL */
+    float* saved;        /* cpab_saved_floats(ndim, 1, nP) floats or NULL */
     const float* grad_out;
     float* grad_data;    /* NULL or zero-filled [C, dsize...] */
     float* grad_theta;   /* [d] */

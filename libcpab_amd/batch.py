@@ -52,7 +52,7 @@ class CpabBatch:
         nP = [int(np.prod(o)) for o in osz]
         n_data = [c * int(np.prod(s)) for c, s in zip(C, dsz)]
         n_out = [c * p for c, p in zip(C, nP)]
-        n_sv = [(k + k * k) * p for k, p in zip(nd, nP)]
+        n_sv = [int(N.lib().cpab_saved_floats(k, 1, p)) for k, p in zip(nd, nP)]
         n_lin = [sum(o) for o in osz]
         ws_b = [int(L.cpab_batch_workspace_bytes(t.ref, H._int_arr(o), 1)) for t, o in zip(self._tess, osz)]
 

@@ -42,6 +42,7 @@
 #define CPAB_HD __host__ __device__
 #include <stdint.h>
 #include <limits.h>
+#include <type_traits>
 
 namespace cpab {
 
@@ -53,6 +54,7 @@ constexpr int NSAMP = 4;        // samples per chunk for root isolation
 constexpr int MAX_ITERS = 512;  // chunks + crossings per point (termination bound)
 constexpr float ZERO_HOP = 1e-7f;
 constexpr int MAX_ZERO_HOPS = 8;
+constexpr int NREC = 2;         // chunks per point the forward records for the backward (2D/3D)
 
 template <int D>
 struct CellTable {
@@ -387,6 +389,95 @@ struct Chunk {
     float q1[D];     // end point
 };
 
+// End of a chunk of duration step that crosses facet bf (-1: none): end point,
+// chunk record and the point's state (ch.M, ch.b and ch.W are set).
+template <int D>
+CPAB_HD __forceinline__ void finish_chunk(PointState<D>& st, const CellTable<D>& tab, float step, int bf,
+                                          Chunk<D>& ch) {
+    const int c = st.c;
+#pragma unroll
+    for (int a = 0; a < D; ++a) ch.q0[a] = st.q[a];
+    series_eval<D>(ch.q0, ch.W, step, ch.q1);
+    ch.h = step;
+    ch.cell = c;
+    ch.facet = bf;
+    ch.next = (bf >= 0) ? tab.nb[c * NF + bf] : c;
+#pragma unroll
+    for (int a = 0; a < D; ++a) st.q[a] = ch.q1[a];
+    st.T = (bf >= 0) ? fmaxf(st.T - step, 0.f) : st.T - step;
+    if (bf < 0 && st.T < 0.f) st.T = 0.f;
+    if (bf >= 0) {
+        st.zero_hops = (step <= ZERO_HOP) ? st.zero_hops + 1 : 0;
+        st.came_from = c;
+        st.c = ch.next;
+    } else if (step > ZERO_HOP) {
+        // time went by inside the cell: the guard against crossing straight
+        // back only covers a run of zero-length crossings
+        st.zero_hops = 0;
+    }
+    st.iters++;
+}
+
+// The first NREC chunks of a point's trajectory: durations and crossed facets.
+// code: bits 0-1 chunks recorded; bits 2+3i..4+3i facet crossed by chunk i, + 1
+// (0: none).
+struct ChunkLog {
+    float h[NREC];
+    int code;
+};
+
+CPAB_HD __forceinline__ void log_chunk(ChunkLog& lg, float h, int facet) {
+    const int n = lg.code & 3;
+#pragma unroll
+    for (int k = 0; k < NREC; ++k)
+        if (k == n) lg.h[k] = h;
+    if (n < NREC) lg.code = ((lg.code & ~3) | (n + 1)) | ((facet + 1) << (2 + 3 * n));
+}
+
+// Sink for the forward's chunk log: durations go straight to memory (h[k] of
+// point i at base[k * stride + i], base common to the wave) as the chunks end;
+// only the facet code stays in a register.  On the device the store is a raw
+// buffer store with a 32-bit lane offset, bounded by the log rows.
+struct LogSink {
+    float* base;
+    int stride;
+    int i;
+    int code;
+#if defined(__HIP_DEVICE_COMPILE__)
+    __amdgpu_buffer_rsrc_t rsrc;
+#endif
+    CPAB_HD void init(float* b, int s, int idx) {
+        base = b;
+        stride = s;
+        i = idx;
+        code = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(b, 0, NREC * s * 4, 0x00020000);
+#endif
+    }
+};
+
+// n: index of the chunk just finished (the point's chunk count - 1).
+CPAB_HD __forceinline__ void sink_chunk(LogSink& lg, int n, float h, int facet) {
+    if (n < NREC) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        // the builtin's data operand is an integer: pass the bits, not the value
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(h), lg.rsrc, (n * lg.stride + lg.i) * 4, 0, 0);
+#else
+        lg.base[(size_t)n * lg.stride + lg.i] = h;
+#endif
+        lg.code = ((lg.code & ~3) | (n + 1)) | ((facet + 1) << (2 + 3 * n));
+    }
+}
+
+CPAB_HD __forceinline__ void logged_chunk(const ChunkLog& lg, int k, float& h, int& facet) {
+    h = lg.h[0];
+#pragma unroll
+    for (int j = 1; j < NREC; ++j)
+        if (k == j) h = lg.h[j];
+    facet = ((lg.code >> (2 + 3 * k)) & 7) - 1;
+}
+
 // ---------------------------------------------------------------------------
 // one chunk of integration, D = 2, 3
 // ---------------------------------------------------------------------------
@@ -573,28 +664,20 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
             }
         }
     }
-    const float step = best;
-#pragma unroll
-    for (int a = 0; a < D; ++a) ch.q0[a] = st.q[a];
-    series_eval<D>(ch.q0, ch.W, step, ch.q1);
-    ch.h = step;
-    ch.cell = c;
-    ch.facet = bf;
-    ch.next = (bf >= 0) ? tab.nb[c * NF + bf] : c;
-#pragma unroll
-    for (int a = 0; a < D; ++a) st.q[a] = ch.q1[a];
-    st.T = (bf >= 0) ? fmaxf(st.T - step, 0.f) : st.T - step;
-    if (bf < 0 && st.T < 0.f) st.T = 0.f;
-    if (bf >= 0) {
-        st.zero_hops = (step <= ZERO_HOP) ? st.zero_hops + 1 : 0;
-        st.came_from = c;
-        st.c = ch.next;
-    } else if (step > ZERO_HOP) {
-        // time went by inside the cell: the guard against crossing straight
-        // back only covers a run of zero-length crossings
-        st.zero_hops = 0;
-    }
-    st.iters++;
+    finish_chunk<D>(st, tab, best, bf, ch);
+}
+
+// Replays a chunk whose duration h and crossed facet bf (-1: none) are known
+// (recorded by the forward): the same series and the same end-of-chunk update
+// as advance, without the facet search, so the state comes out bit for bit
+// as in the forward.
+template <int D>
+CPAB_HD __forceinline__ void replay(PointState<D>& st, const CellTable<D>& tab, float h, int bf, Chunk<D>& ch) {
+    load_affine<D>(tab, st.c, ch.M, ch.b);
+    float v0[D];
+    affine_apply<D>(ch.M, ch.b, st.q, v0);
+    series<D>(ch.M, v0, ch.W);
+    finish_chunk<D>(st, tab, h, bf, ch);
 }
 
 // ---------------------------------------------------------------------------
@@ -818,10 +901,11 @@ CPAB_HD __forceinline__ bool point_active(const PointState<D>& st) {
     return st.T > 0.f && st.iters < MAX_ITERS;
 }
 
-// One chunk of pass 1: advance the point and carry J = dq(T)/dq0.
-template <int D, bool WITHJ>
+// One chunk of pass 1: advance the point and carry J = dq(T)/dq0; with LOG
+// (2D/3D) the chunk goes into lg (a ChunkLog or a LogSink).
+template <int D, bool WITHJ, bool LOG = false, class Log = ChunkLog>
 CPAB_HD __forceinline__ void step_point(PointState<D>& st, const CellTable<D>& tab, int nx, float tol0,
-                                        float (&J)[D][D]) {
+                                        float (&J)[D][D], Log& lg) {
     if constexpr (D == 1) {
         Chunk1 ch;
         advance1(st, tab, nx, ch);
@@ -854,6 +938,13 @@ CPAB_HD __forceinline__ void step_point(PointState<D>& st, const CellTable<D>& t
                 for (int j = 0; j < D; ++j) J[r][j] = Jn[r][j];
             if (ch.facet >= 0) saltation_fwd<D>(tab, ch, J);
         }
+        // logged last, when the chunk's temporaries are dead
+        if constexpr (LOG) {
+            if constexpr (std::is_same<Log, LogSink>::value)
+                sink_chunk(lg, st.iters - 1, ch.h, ch.facet);
+            else
+                log_chunk(lg, ch.h, ch.facet);
+        }
     }
 }
 
@@ -865,16 +956,25 @@ CPAB_HD __forceinline__ void identity(float (&J)[D][D]) {
         for (int k = 0; k < D; ++k) J[r][k] = (r == k) ? 1.f : 0.f;
 }
 
-// Integrate one point for time T (pass 1 if WITHJ).
-template <int D, bool WITHJ>
+// Integrate one point for time T (pass 1 if WITHJ); with LOG its first NREC
+// chunks go into lg for the backward to replay.
+template <int D, bool WITHJ, bool LOG = false, class Log = ChunkLog>
 CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
-                                       float (&J)[D][D]) {
+                                       float (&J)[D][D], Log& lg) {
     PointState<D> st;
     init_point<D>(st, q, c0, T);
     if constexpr (WITHJ) identity<D>(J);
-    while (point_active<D>(st)) step_point<D, WITHJ>(st, tab, nx, tol0, J);
+    lg.code = 0;
+    while (point_active<D>(st)) step_point<D, WITHJ, LOG, Log>(st, tab, nx, tol0, J, lg);
 #pragma unroll
     for (int a = 0; a < D; ++a) q[a] = st.q[a];
+}
+
+template <int D, bool WITHJ>
+CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+                                       float (&J)[D][D]) {
+    ChunkLog lg;
+    integrate<D, WITHJ, false>(q, c0, T, tab, nx, tol0, J, lg);
 }
 
 // ---------------------------------------------------------------------------

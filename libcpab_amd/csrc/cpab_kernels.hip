@@ -193,15 +193,21 @@ __device__ __forceinline__ CellTable<D> stage_tables(float* smem, const float* A
     return t;
 }
 
+// Rows of the fused forward's saved state per point: warped point, q-space
+// Jacobian and, in 2D/3D, the chunk log (NREC durations + the facet code).
+__host__ __device__ constexpr int saved_rows(int D) { return D + D * D + (D > 1 ? NREC + 1 : 0); }
+
 template <int D>
 __host__ __device__ constexpr size_t table_floats(int nC) {
     return (size_t)nC * D * (D + 1) + ((nC + 3) & ~3) + (D > 1 ? (size_t)nC * NF * 4 + (size_t)nC * NF : 0);
 }
 
 // Pass 2 over one point per lane (wave-convergent); accumulates into acc.
+// The chunks recorded in lg by the forward are replayed (no facet search);
+// the rest are found again.
 template <int D>
 __device__ __forceinline__ void pass2(bool valid, const float (&q0)[D], int c0, float T, float (&mu)[D],
-                                      const CellTable<D>& tab, const Geo& g, float* acc) {
+                                      const CellTable<D>& tab, const Geo& g, float* acc, const ChunkLog& lg) {
     constexpr int P = D * (D + 1);
     PointState<D> st;
 #pragma unroll
@@ -211,6 +217,7 @@ __device__ __forceinline__ void pass2(bool valid, const float (&q0)[D], int c0, 
     st.came_from = -1;
     st.zero_hops = 0;
     st.iters = 0;
+    int nrec = lg.code & 3;
     while (true) {
         const bool active = st.T > 0.f && st.iters < MAX_ITERS;
         if (__ballot(active) == 0) break;
@@ -234,7 +241,18 @@ __device__ __forceinline__ void pass2(bool valid, const float (&q0)[D], int c0, 
                 }
             } else {
                 Chunk<D> ch;
-                advance<D>(st, tab, g.tol0, ch);
+                float h = 0.f;
+                int bf = -1;
+                if (st.iters < nrec) {
+                    logged_chunk(lg, st.iters, h, bf);
+                    // only a chunk the forward can have made is replayed (a
+                    // saved state from another call is searched again instead)
+                    if (!(h >= 0.f && h <= st.T) || bf >= NF || (bf >= 0 && tab.nb[st.c * NF + bf] < 0)) nrec = 0;
+                }
+                if (st.iters < nrec)
+                    replay<D>(st, tab, h, bf, ch);
+                else
+                    advance<D>(st, tab, g.tol0, ch);
                 chunk_grad<D>(ch, mu, G);
                 cell = ch.cell;
                 if (ch.facet >= 0) saltation_adj<D>(tab, ch, mu);
@@ -427,8 +445,10 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
     }
 }
 
-// meshgrid -> integrate -> interpolate.  saved (nullable): [n_theta, D + D*D, nP]
-// holds the warped point (x-space) and the q-space Jacobian for the backward.
+// meshgrid -> integrate -> interpolate.  saved (nullable): [n_theta, rows, nP]
+// holds the warped point (x-space), the q-space Jacobian and, in 2D/3D, the
+// log of the first NREC chunks (durations, then the packed facet code) that
+// the backward replays instead of searching again; rows = D + D*D (+ NREC + 1).
 // One tile (tile_t) of one instance's fused forward; data / out / sv point at
 // the instance's slices.
 template <int D, bool SAVE>
@@ -452,7 +472,10 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
         }
         const int c0 = find_cell<D>(x, nc);
         float J[D][D];
-        integrate<D, SAVE>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+        // with saved state (2D/3D) the chunk durations go straight to their rows
+        LogSink lg;
+        if constexpr (SAVE && D > 1) lg.init(sv + (size_t)(D + D * D) * L.nP, L.nP, t);
+        integrate<D, SAVE, SAVE && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
         float y[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) y[a] = q[a] * rnc[a];
@@ -464,6 +487,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
             for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
                 for (int k = 0; k < D; ++k) sv[(size_t)(D + r2 * D + k) * L.nP + t] = J[r2][k];
+            if constexpr (D > 1) sv[(size_t)(D + D * D + NREC) * L.nP + t] = __int_as_float(lg.code);
         }
         int i0[3], i1[3];
         float fd[3];
@@ -483,7 +507,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data(Launch L, Geo g, Src s
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * g.nC * D * (D + 1), g);
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
     fused_fwd_tile<D, SAVE>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
-                            out + (size_t)th * im.C * L.nP, SAVE ? saved + (size_t)th * (D + D * D) * L.nP : nullptr);
+                            out + (size_t)th * im.C * L.nP, SAVE ? saved + (size_t)th * saved_rows(D) * L.nP : nullptr);
 }
 
 template <int D>
@@ -522,6 +546,8 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
         const bool valid = p < L.nP;
         float q0[D], mu[D];
         int c0 = 0;
+        ChunkLog lg;
+        lg.code = 0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             q0[a] = 0.f;
@@ -546,7 +572,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
                 float q[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) q[a] = q0[a];
-                integrate<D, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+                integrate<D, true, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
             }
             float w[D];
             const float* go = gout + (size_t)th * D * L.nP;
@@ -565,7 +591,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
                 for (int a = 0; a < D; ++a) gp[(size_t)a * L.nP + p] = mu[a] * (float)nc[a];
             }
         }
-        pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc);
+        pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc, lg);
     }
     write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
 }
@@ -659,6 +685,8 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
         if (use_box && !valid) code[r * BLOCK + threadIdx.x] = -1;
         float q0[D], mu[D];
         int c0 = 0;
+        ChunkLog lg;
+        lg.code = 0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             q0[a] = 0.f;
@@ -682,11 +710,16 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
                 for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
                     for (int k = 0; k < D; ++k) J[r2][k] = sv[(size_t)(D + r2 * D + k) * L.nP + t];
+                if constexpr (D > 1) {
+#pragma unroll
+                    for (int k = 0; k < NREC; ++k) lg.h[k] = sv[(size_t)(D + D * D + k) * L.nP + t];
+                    lg.code = __float_as_int(sv[(size_t)(D + D * D + NREC) * L.nP + t]);
+                }
             } else {
                 float q[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) q[a] = q0[a];
-                integrate<D, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+                integrate<D, true, true>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
 #pragma unroll
                 for (int a = 0; a < D; ++a) y[a] = q[a] * rnc[a];
             }
@@ -723,7 +756,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
                 mu[k] = acc2;
             }
         }
-        pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc);
+        pass2<D>(valid, q0, c0, L.t_end, mu, tab, g, wacc, lg);
     }
     if (use_box) {
         __syncthreads();
@@ -782,7 +815,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
     fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
                              im.gdata ? im.gdata + (size_t)th * im.C * dstride : nullptr,
-                             gout + (size_t)th * im.C * L.nP, HAVES ? saved + (size_t)th * (D + D * D) * L.nP : nullptr,
+                             gout + (size_t)th * im.C * L.nP, HAVES ? saved + (size_t)th * saved_rows(D) * L.nP : nullptr,
                              acc + (threadIdx.x >> 6) * nC * P, box_cap ? acc + WAVES * nC * P : nullptr, box_cap);
     write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
 }
@@ -1549,7 +1582,7 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
     return status_from_hip();
 }
 
-size_t cpab_saved_floats(int ndim, int n_theta, int nP) { return (size_t)n_theta * (ndim + ndim * ndim) * nP; }
+size_t cpab_saved_floats(int ndim, int n_theta, int nP) { return (size_t)n_theta * saved_rows(ndim) * nP; }
 
 size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* outsize) {
     if (check_tess(t) != CPAB_OK || n_theta < 0 || !outsize) return 0;
@@ -1691,7 +1724,7 @@ int cpab_transform_data_fwd_plain(const cpab_tess_t* t, const float* theta, int 
     if ((rc = img_from(t->ndim, data, C, dsize, im))) return rc;
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     const size_t dstride = (size_t)im.s0 * im.s1 * im.s2;
-    const size_t sv = (size_t)(t->ndim + t->ndim * t->ndim) * nP;
+    const size_t sv = (size_t)saved_rows(t->ndim) * nP;
     if (ws_bytes < cpab_data_workspace_bytes(t, 1, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
     for (int i = 0; i < n_theta; ++i) {
         rc = cpab_transform_data_fwd(t, theta + (size_t)i * t->d, 1, data + (size_t)i * C * dstride, C, dsize, outsize,
@@ -1714,7 +1747,7 @@ int cpab_transform_data_bwd_plain(const cpab_tess_t* t, const float* theta, int 
     if ((rc = img_from(t->ndim, data, C, dsize, im))) return rc;
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     const size_t dstride = (size_t)im.s0 * im.s1 * im.s2;
-    const size_t sv = (size_t)(t->ndim + t->ndim * t->ndim) * nP;
+    const size_t sv = (size_t)saved_rows(t->ndim) * nP;
     if (ws_bytes < cpab_data_workspace_bytes(t, 1, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
     for (int i = 0; i < n_theta; ++i) {
         rc = cpab_transform_data_bwd(t, theta + (size_t)i * t->d, 1, data + (size_t)i * C * dstride, C, dsize, outsize,

@@ -4,6 +4,7 @@
 // the library never loads this.
 #include <math.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "../libcpab_amd/csrc/cpab_device.hpp"
 
@@ -94,9 +95,101 @@ extern "C" int host_integrate_state2(const int* nc, int nC, const float* Aq, con
     st.came_from = came_from;
     st.zero_hops = zero_hops;
     float J[2][2];
-    while (point_active<2>(st)) step_point<2, false>(st, tab, nc[0], tol0, J);
+    ChunkLog lg;
+    while (point_active<2>(st)) step_point<2, false>(st, tab, nc[0], tol0, J, lg);
     q[0] = st.q[0];
     q[1] = st.q[1];
     free(inv);
     return 0;
+}
+
+// The backward's replay of the forward's chunk log: integrates every point
+// with the log on, then again replaying the logged chunks (the rest by the
+// facet search) and counts the points whose chunks (cell, duration, facet,
+// end point) or end state differ in any bit.  Returns the mismatch count.
+template <int D>
+static int replay_mismatches(const int* nc, int nC, const float* Aq, const float* planes, const int* nbr,
+                             const float* pts, int nP, float t_end, float tol0, int* n_logged) {
+    constexpr int P = D * (D + 1);
+    float* inv = (float*)malloc(sizeof(float) * nC);
+    for (int k = 0; k < nC; ++k) {
+        float s = 0.f;
+        for (int r = 0; r < D; ++r)
+            for (int j = 0; j < D; ++j) s += Aq[k * P + r * (D + 1) + j] * Aq[k * P + r * (D + 1) + j];
+        inv[k] = s > 0.f ? 1.0f / sqrtf(s) : INFINITY;
+    }
+    CellTable<D> tab;
+    tab.A = Aq;
+    tab.inv_n = inv;
+    tab.pl = reinterpret_cast<const float4*>(planes);
+    tab.nb = nbr;
+    int bad = 0, logged = 0;
+    for (int p = 0; p < nP; ++p) {
+        float x[D], q[D];
+        for (int a = 0; a < D; ++a) {
+            x[a] = pts[(size_t)a * nP + p];
+            q[a] = x[a] * (float)nc[a];
+        }
+        const int c0 = find_cell<D>(x, nc);
+        // forward: search, keep every chunk
+        PointState<D> s1, s2;
+        init_point<D>(s1, q, c0, t_end);
+        init_point<D>(s2, q, c0, t_end);
+        ChunkLog lg;
+        lg.code = 0;
+        Chunk<D> c1[16];
+        int n1 = 0;
+        while (point_active<D>(s1)) {
+            Chunk<D> ch;
+            advance<D>(s1, tab, tol0, ch);
+            log_chunk(lg, ch.h, ch.facet);
+            if (n1 < 16) c1[n1] = ch;
+            n1++;
+        }
+        const int nrec = lg.code & 3;
+        logged += nrec;
+        // the forward's memory sink records the same log
+        {
+            float hs[NREC] = {-1.f, -1.f};
+            LogSink sk;
+            sk.init(hs, 1, 0);
+            float qs[D], Js[D][D];
+            for (int a = 0; a < D; ++a) qs[a] = q[a];
+            integrate<D, true, true, LogSink>(qs, c0, t_end, tab, nc[0], tol0, Js, sk);
+            bool ok = sk.code == lg.code;
+            for (int k = 0; k < nrec; ++k) ok = ok && memcmp(&hs[k], &lg.h[k], sizeof(float)) == 0;
+            bad += !ok;
+        }
+        int n2 = 0;
+        bool same = true;
+        while (point_active<D>(s2)) {
+            Chunk<D> ch;
+            if (s2.iters < nrec) {
+                float h;
+                int bf;
+                logged_chunk(lg, s2.iters, h, bf);
+                replay<D>(s2, tab, h, bf, ch);
+            } else {
+                advance<D>(s2, tab, tol0, ch);
+            }
+            if (n2 < 16 && n2 < n1) {
+                same = same && ch.cell == c1[n2].cell && ch.facet == c1[n2].facet && ch.next == c1[n2].next &&
+                       memcmp(&ch.h, &c1[n2].h, sizeof(float)) == 0 && memcmp(ch.q1, c1[n2].q1, sizeof(ch.q1)) == 0;
+            }
+            n2++;
+        }
+        same = same && n1 == n2 && memcmp(s1.q, s2.q, sizeof(s1.q)) == 0 && s1.c == s2.c &&
+               memcmp(&s1.T, &s2.T, sizeof(float)) == 0;
+        bad += !same;
+    }
+    free(inv);
+    *n_logged = logged;
+    return bad;
+}
+
+extern "C" int host_replay_mismatches(int ndim, const int* nc, int nC, const float* Aq, const float* planes,
+                                      const int* nbr, const float* pts, int nP, float t_end, float tol0,
+                                      int* n_logged) {
+    if (ndim == 2) return replay_mismatches<2>(nc, nC, Aq, planes, nbr, pts, nP, t_end, tol0, n_logged);
+    return replay_mismatches<3>(nc, nC, Aq, planes, nbr, pts, nP, t_end, tol0, n_logged);
 }

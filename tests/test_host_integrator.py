@@ -249,3 +249,33 @@ def test_host_3d_starts_on_faces_near_tangency(nc, scale):
         ref = O.cf_forward(3, nc, A[k:k + 1], planes, nbr, grid.astype(np.float64))
         err = np.abs(got - ref).max()
         assert err <= 1e-5, (k, err, grid[:, np.abs(got - ref).max(1).argmax()])
+
+
+@pytest.mark.parametrize("nc,zb,scale", [([4, 4], True, 0.3), ([4, 4], True, 1.5), ([3, 3], False, 0.6),
+                                          ([2, 2, 2], True, 0.5), ([3, 3, 3], True, 1.0)])
+def test_host_replay_of_the_chunk_log_is_bit_exact(nc, zb, scale):
+    """The backward replays the first chunks the forward logged (duration and
+    crossed facet) instead of searching for them again; every replayed chunk,
+    and the end state, must equal the forward's bit for bit.  Points on
+    facets, vertices and the boundary included."""
+    t = make_tessellation(nc, zero_boundary=zb)
+    planes, nbr = t.facet_table()
+    nd = len(nc)
+    rng = np.random.default_rng(7 + sum(nc))
+    theta = rng.standard_normal((2, t.B.shape[1])) * scale
+    side = {2: 41, 3: 13}[nd]
+    grid = np.concatenate([O.uniform_meshgrid([side] * nd), rng.uniform(0, 1, size=(nd, 500))], 1)
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    S = np.array(nc, dtype=np.float64)
+    for k in range(theta.shape[0]):
+        A = O.expand_A(t.B, theta[k:k + 1], nd)[0]
+        Aq = A * np.concatenate([S[:, None] / S[None, :], S[:, None]], 1)
+        Aq = np.ascontiguousarray(Aq.reshape(-1, nd * (nd + 1)).astype(np.float32))
+        ncarr = np.array(list(nc) + [1] * (3 - nd), np.int32)
+        logged = ctypes.c_int(0)
+        bad = _lib().host_replay_mismatches(nd, _ptr(ncarr), nbr.shape[0], _ptr(Aq),
+                                            _ptr(np.ascontiguousarray(planes, np.float32)),
+                                            _ptr(np.ascontiguousarray(nbr, np.int32)), _ptr(grid), grid.shape[1],
+                                            ctypes.c_float(1.0), ctypes.c_float(2e-6 * max(nc)), ctypes.byref(logged))
+        assert bad == 0, bad
+        assert logged.value >= grid.shape[1]

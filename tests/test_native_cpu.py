@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = N.lib()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.cpab_abi_version() == 2
+    assert lib.cpab_abi_version() == 3
     assert lib.cpab_status_string(0) == b"ok"
     assert b"workspace" in lib.cpab_status_string(3)
 
@@ -86,7 +86,9 @@ def test_workspace_and_saved_sizes():
     D = 64 * 6
     ws = L.cpab_workspace_bytes(ctypes.byref(t), 256, 65536)
     assert ws >= 256 * D * 4
-    assert L.cpab_saved_floats(2, 256, 65536) == 256 * 6 * 65536
+    assert L.cpab_saved_floats(2, 256, 65536) == 256 * (6 + 3) * 65536
+    assert L.cpab_saved_floats(3, 16, 4096) == 16 * (12 + 3) * 4096
+    assert L.cpab_saved_floats(1, 8, 1024) == 8 * 2 * 1024
     # transform_data: A_q plus one dL/dA partial per 2D tile (16 x 32 points in the backward)
     osz = (ctypes.c_int * 3)(256, 256, 1)
     wd = L.cpab_data_workspace_bytes(ctypes.byref(t), 256, osz)
