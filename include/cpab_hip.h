@@ -72,13 +72,19 @@ size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* o
  * first two chunks, which the backward replays without searching again. */
 size_t cpab_saved_floats(int ndim, int n_theta, int nP);
 
+/* Floats in cpab_transform_fwd's saved state (jac) for cpab_transform_bwd:
+ * n_theta * (ndim^2 + (ndim > 1 ? 3 : 0)) * nP -- the q-space Jacobians and,
+ * in 2D/3D, each point's chunk log. */
+size_t cpab_jac_floats(int ndim, int n_theta, int nP);
+
 /* A_q = B theta per cell, in q-space.  Replaces the basis expansion of
  * libcpab/pytorch/transformer.py:146-150. */
 int cpab_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, float* Aq, hipStream_t stream);
 
 /* Closed-form integration of the points for time t_end (1 in the Cpab API).
- * jac (nullable) receives the q-space Jacobian [n_theta, ndim*ndim, nP] used by
- * cpab_transform_bwd to skip its first pass.  Replaces cpab_gpu.forward,
+ * jac (nullable, cpab_jac_floats) receives per theta the q-space Jacobian
+ * [ndim*ndim, nP] and (2D/3D) the chunk log [3, nP], which let
+ * cpab_transform_bwd skip its first pass and replay the chunks of its second.  Replaces cpab_gpu.forward,
  * libcpab/pytorch/transformer_cuda.cpp:17-41 (+ the Trels expm of
  * libcpab/pytorch/transformer.py:152-155). */
 int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,

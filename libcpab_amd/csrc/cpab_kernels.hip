@@ -196,6 +196,9 @@ __device__ __forceinline__ CellTable<D> stage_tables(float* smem, const float* A
 // Rows of the fused forward's saved state per point: warped point, q-space
 // Jacobian and, in 2D/3D, the chunk log (NREC durations + the facet code).
 __host__ __device__ constexpr int saved_rows(int D) { return D + D * D + (D > 1 ? NREC + 1 : 0); }
+// Rows of the grid transform's saved state (the jac buffer): q-space Jacobian
+// and, in 2D/3D, the chunk log.
+__host__ __device__ constexpr int jac_rows(int D) { return D * D + (D > 1 ? NREC + 1 : 0); }
 
 template <int D>
 __host__ __device__ constexpr size_t table_floats(int nC) {
@@ -431,12 +434,15 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
         }
         const int c0 = find_cell<D>(x, nc);
         float J[D][D];
-        integrate<D, WITHJ>(q, c0, L.t_end, tab, g.nc0, g.tol0, J);
+        float* jo = jac + (size_t)th * jac_rows(D) * L.nP;
+        LogSink lg;
+        if constexpr (WITHJ && D > 1) lg.init(jo + (size_t)D * D * L.nP, L.nP, p);
+        integrate<D, WITHJ, WITHJ && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
         float* o = out + (size_t)th * D * L.nP;
 #pragma unroll
         for (int a = 0; a < D; ++a) o[(size_t)a * L.nP + p] = q[a] * rnc[a];
         if constexpr (WITHJ) {
-            float* jo = jac + (size_t)th * D * D * L.nP;
+            if constexpr (D > 1) jo[(size_t)(D * D + NREC) * L.nP + p] = __int_as_float(lg.code);
 #pragma unroll
             for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
@@ -563,11 +569,16 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
             c0 = find_cell<D>(x, nc);
             float J[D][D];
             if constexpr (HAVEJ) {
-                const float* jo = jac + (size_t)th * D * D * L.nP;
+                const float* jo = jac + (size_t)th * jac_rows(D) * L.nP;
 #pragma unroll
                 for (int r2 = 0; r2 < D; ++r2)
 #pragma unroll
                     for (int k = 0; k < D; ++k) J[r2][k] = jo[(size_t)(r2 * D + k) * L.nP + p];
+                if constexpr (D > 1) {
+#pragma unroll
+                    for (int k = 0; k < NREC; ++k) lg.h[k] = jo[(size_t)(D * D + k) * L.nP + p];
+                    lg.code = __float_as_int(jo[(size_t)(D * D + NREC) * L.nP + p]);
+                }
             } else {
                 float q[D];
 #pragma unroll
@@ -1583,6 +1594,8 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
 }
 
 size_t cpab_saved_floats(int ndim, int n_theta, int nP) { return (size_t)n_theta * saved_rows(ndim) * nP; }
+
+size_t cpab_jac_floats(int ndim, int n_theta, int nP) { return (size_t)n_theta * jac_rows(ndim) * nP; }
 
 size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* outsize) {
     if (check_tess(t) != CPAB_OK || n_theta < 0 || !outsize) return 0;

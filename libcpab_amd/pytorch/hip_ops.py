@@ -116,7 +116,8 @@ class TransformGrid(torch.autograd.Function):
         jac = None
         store = getattr(params, "store_jacobian", True) if store_jac is None else store_jac
         if need and store:
-            jac = torch.empty(n_theta, ndim * ndim, nP, device=theta.device, dtype=torch.float32)
+            jac = torch.empty(int(N.lib().cpab_jac_floats(ndim, n_theta, nP)), device=theta.device,
+                              dtype=torch.float32)
         ws, nb = dev.workspace(n_theta, nP)
         N.check(N.lib().cpab_transform_fwd(dev.ref, _ptr(theta), n_theta, _ptr(points), nP, per_theta,
                                            float(t_end), _ptr(out), _ptr(jac), _ptr(ws), nb, _stream(theta.device)))

@@ -89,6 +89,8 @@ def test_workspace_and_saved_sizes():
     assert L.cpab_saved_floats(2, 256, 65536) == 256 * (6 + 3) * 65536
     assert L.cpab_saved_floats(3, 16, 4096) == 16 * (12 + 3) * 4096
     assert L.cpab_saved_floats(1, 8, 1024) == 8 * 2 * 1024
+    assert L.cpab_jac_floats(2, 256, 65536) == 256 * (4 + 3) * 65536
+    assert L.cpab_jac_floats(1, 8, 1024) == 8 * 1024
     # transform_data: A_q plus one dL/dA partial per 2D tile (16 x 32 points in the backward)
     osz = (ctypes.c_int * 3)(256, 256, 1)
     wd = L.cpab_data_workspace_bytes(ctypes.byref(t), 256, osz)
