@@ -100,6 +100,17 @@ int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, co
                        int per_theta_grid, float t_end, const float* grad_out, const float* jac, float* grad_theta,
                        float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream);
 
+/* Validation: the cell sequence of each point's trajectory under the same
+ * integration as cpab_transform_fwd, as segments of positive duration
+ * (consecutive chunks in one cell merge, zero-length visits at vertices are
+ * dropped).  cells [n_theta, nP, max_segs] receives the first max_segs cells
+ * (-1 padded), nseg [n_theta, nP] the number of segments.  The reference has
+ * no counterpart (its numpy path's cell indices are what it is compared
+ * against, libcpab/numpy/transformer.py:22-61). */
+int cpab_transform_cells(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
+                         int per_theta_grid, float t_end, int max_segs, int* cells, int* nseg, void* workspace,
+                         size_t ws_bytes, hipStream_t stream);
+
 /* Linear interpolation of data at grid [N, ndim, nP], nP = prod(outsize).
  * Replaces libcpab/pytorch/interpolation.py:11-171 (interpolate1D/2D/3D). */
 int cpab_interpolate_fwd(int ndim, const float* data, int N, int C, const int* dsize, const float* grid,

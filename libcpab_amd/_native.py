@@ -43,6 +43,8 @@ _SIGNATURES = {
                                     _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "cpab_transform_bwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
                                     _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_cells": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
+                                      _c_int, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "cpab_interpolate_fwd": (_c_int, [_c_int, _c_ptr, _c_int, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr]),
     "cpab_interpolate_bwd": (_c_int, [_c_int, _c_ptr, _c_int, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                       _c_ptr, _c_ptr]),

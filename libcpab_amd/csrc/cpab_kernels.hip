@@ -451,6 +451,69 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
     }
 }
 
+// Validation: the cells each point's trajectory passes through, as segments
+// of positive duration (consecutive chunks in one cell merge; zero-length
+// visits at vertices are not segments).  cells [n_theta, nP, max_segs] gets
+// the first max_segs (-1 padded), nseg [n_theta, nP] the full count.  Same
+// integration as k_transform.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_transform_cells(Launch L, Geo g, Src s, const float* __restrict__ Aq,
+                                                           int max_segs, int* __restrict__ cells,
+                                                           int* __restrict__ nseg) {
+    extern __shared__ float smem[];
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int th = lb / L.tiles, tile = lb % L.tiles;
+    if (th >= L.n_theta) return;
+    const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * g.nC * D * (D + 1), g);
+    const int nc[3] = {g.nc0, g.nc1, g.nc2};
+    const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
+    for (int r = 0; r < L.ppt; ++r) {
+        const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
+        if (p >= L.nP) break;
+        float x[D], q[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            x[a] = pts[(size_t)a * L.nP + p];
+            q[a] = x[a] * (float)nc[a];
+        }
+        PointState<D> st;
+        init_point<D>(st, q, find_cell<D>(x, nc), L.t_end);
+        int* out = cells + ((size_t)th * L.nP + p) * max_segs;
+        int n = 0;
+        float seg_t = 0.f;
+        while (point_active<D>(st)) {
+            int cell, facet;
+            float h;
+            if constexpr (D == 1) {
+                Chunk1 ch;
+                advance1(st, tab, g.nc0, ch);
+                cell = ch.cell;
+                facet = ch.facet;
+                h = ch.h;
+            } else {
+                Chunk<D> ch;
+                advance<D>(st, tab, g.tol0, ch);
+                cell = ch.cell;
+                facet = ch.facet;
+                h = ch.h;
+            }
+            seg_t += h;
+            if (facet >= 0) {
+                if (seg_t > 0.f) {
+                    if (n < max_segs) out[n] = cell;
+                    ++n;
+                }
+                seg_t = 0.f;
+            } else if (!point_active<D>(st) && seg_t > 0.f) {
+                if (n < max_segs) out[n] = cell;
+                ++n;
+            }
+        }
+        for (int k = n; k < max_segs; ++k) out[k] = -1;
+        nseg[(size_t)th * L.nP + p] = n;
+    }
+}
+
 // meshgrid -> integrate -> interpolate.  saved (nullable): [n_theta, rows, nP]
 // holds the warped point (x-space), the q-space Jacobian and, in 2D/3D, the
 // log of the first NREC chunks (durations, then the packed facet code) that
@@ -1485,6 +1548,38 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
         hipLaunchKernelGGL((k_transform<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, Aq, out, jac);
     CPAB_DISPATCH_D(t->ndim, LAUNCH_TF)
 #undef LAUNCH_TF
+    return status_from_hip();
+}
+
+int cpab_transform_cells(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
+                         int per_theta_grid, float t_end, int max_segs, int* cells, int* nseg, void* workspace,
+                         size_t ws_bytes, hipStream_t stream) {
+    int rc = check_tess(t);
+    if (rc) return rc;
+    if (n_theta < 0 || nP < 0 || max_segs < 0) return CPAB_ERR_ARG;
+    if (n_theta == 0 || nP == 0) return CPAB_OK;
+    if (!theta || !points || !nseg || (max_segs > 0 && !cells)) return CPAB_ERR_ARG;
+    if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
+    const size_t lds = lds_tables(t->ndim, t->nC);
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    float* Aq = static_cast<float*>(workspace);
+    if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
+    Launch L;
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = 1;
+    L.ty = 0;
+    L.tiles = (nP + BLOCK - 1) / BLOCK;
+    L.t_end = t_end;
+    Geo g = make_geo(t);
+    Src s = {};
+    s.pts = points;
+    s.per_theta = per_theta_grid;
+    dim3 grid(L.tiles * n_theta);
+#define LAUNCH_TC(DD) \
+    hipLaunchKernelGGL((k_transform_cells<DD>), grid, dim3(BLOCK), lds, stream, L, g, s, Aq, max_segs, cells, nseg);
+    CPAB_DISPATCH_D(t->ndim, LAUNCH_TC)
+#undef LAUNCH_TC
     return status_from_hip();
 }
 

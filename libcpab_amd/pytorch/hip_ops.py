@@ -151,6 +151,26 @@ class TransformGrid(torch.autograd.Function):
 # --------------------------------------------------------------------------
 # interpolate
 # --------------------------------------------------------------------------
+def cell_sequences(points, theta, params, max_segs=8, t_end=1.0):
+    """Validation: the cells each trajectory passes through (segments of
+    positive duration), cells [n_theta, nP, max_segs] (-1 padded) and the
+    segment count nseg [n_theta, nP], from the same integration as
+    TransformGrid."""
+    _require_gpu_f32("points", points)
+    _require_gpu_f32("theta", theta)
+    dev = device_tess(params, theta.device)
+    theta = theta.contiguous()
+    points = points.contiguous()
+    n_theta, nP = theta.shape[0], points.shape[-1]
+    cells = torch.empty(n_theta, nP, max_segs, device=theta.device, dtype=torch.int32)
+    nseg = torch.empty(n_theta, nP, device=theta.device, dtype=torch.int32)
+    ws, nb = dev.workspace(n_theta, nP)
+    N.check(N.lib().cpab_transform_cells(dev.ref, _ptr(theta), n_theta, _ptr(points), nP,
+                                         int(points.dim() == 3), float(t_end), max_segs, _ptr(cells), _ptr(nseg),
+                                         _ptr(ws), nb, _stream(theta.device)))
+    return cells, nseg
+
+
 class Interpolate(torch.autograd.Function):
     @staticmethod
     def forward(ctx, data, grid, ndim, outsize):
