@@ -235,6 +235,31 @@ def test_reference_scheme_converges_to_exact(name, scale):
     assert errs[0] / errs[2] > 300  # second order in 1/nstep
 
 
+@pytest.mark.parametrize("name,scale", [("2d_3x3", 0.5), ("1d_10", 0.8), ("3d_2x2x2", 0.3), ("2d_4x4", 0.3)])
+def test_reference_gradient_converges_to_the_exact_gradient(name, scale):
+    """The reference's own gradient -- midpoint-RK2 sensitivities over nstep
+    steps (libcpab/core/cpab_ops.cpp:292-367, restated bit-exactly by
+    step_backward_f32) -- agrees with the exact gradient within its
+    discretisation error at the default 50 steps and approaches it as nstep
+    grows.  Its steps take the cell at their start, so steps that straddle a
+    facet make the convergence about first order and uneven."""
+    b, nc, ndim, A, theta = _setup(name, 1, scale, seed=21)
+    rng = np.random.default_rng(21)
+    rng.standard_normal((1, b["B"].shape[1]))
+    pts = rng.uniform(0.05, 0.95, size=(ndim, 60))
+    w = rng.standard_normal((1, ndim, 60))
+    gA, _ = O.cf_backward(ndim, nc, A, b["planes"], b["nbr"], pts, w)
+    exact = (gA.reshape(1, -1) @ b["B"])[0]
+    Bs = O.bases_Bs(b["B"], A.shape[1], ndim)
+    errs = {}
+    for n in (12, 50, 400):
+        J = O.step_backward_f32(ndim, nc, n, pts, A, Bs).astype(np.float64)
+        g = (J[:, 0] * w[0][None]).sum((1, 2))
+        errs[n] = np.abs(g - exact).max() / np.abs(exact).max()
+    assert errs[50] < 2e-2, errs
+    assert errs[400] < errs[12] / 10, errs
+
+
 @pytest.mark.parametrize("name,scale", [("2d_3x3", 0.5), ("3d_2x2x2", 0.3), ("1d_50", 0.6)])
 def test_cell_sequences_match_brute_force(name, scale):
     """Cells visited by the exact integrator equal those of a fine RK4
