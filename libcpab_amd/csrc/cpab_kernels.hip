@@ -1107,7 +1107,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_A(const cpab_instance_t* __rest
     static_cast<float*>(I->ws)[i] = acc * sc;
 }
 
-template <bool SAVE>
+// MAXD: the largest ndim among the instances (2 or 3); without 3D instances
+// the 3D code is not compiled in, which keeps the register count (and the
+// occupancy) of the 1D/2D paths.
+template <bool SAVE, int MAXD>
 __global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __restrict__ inst,
                                                      const int* __restrict__ item_inst, int n_items, int ppt,
                                                      float t_end, int* counter) {
@@ -1137,10 +1140,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __re
                 const CellTable<2> tab = restage ? stage_tables<2>(smem, Aq, g) : table_view<2>(smem, g.nC);
                 fused_fwd_tile<2, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
             } break;
-            default: {
-                const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC);
-                fused_fwd_tile<3, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
-            } break;
+            default:
+                if constexpr (MAXD == 3) {
+                    const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC);
+                    fused_fwd_tile<3, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+                }
+                break;
         }
     }
 }
@@ -1161,8 +1166,8 @@ __device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, flo
     write_partials<D>(acc, part + (size_t)tile * nCP, nCP);
 }
 
-template <bool HAVES>
-__global__ __launch_bounds__(BLOCK) void k_batch_bwd(const cpab_instance_t* __restrict__ inst,
+template <bool HAVES, int MAXD>
+__global__ __launch_bounds__(BLOCK, MAXD == 2 ? 4 : 1) void k_batch_bwd(const cpab_instance_t* __restrict__ inst,
                                                      const int* __restrict__ item_inst, int n_items, int ppt,
                                                      float t_end, int* counter, int acc_offset, int box_offset,
                                                      int box_cap) {
@@ -1187,7 +1192,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_bwd(const cpab_instance_t* __re
         switch (g.ndim) {
             case 1: batch_bwd_item<1, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
             case 2: batch_bwd_item<2, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
-            default: batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
+            default:
+                if constexpr (MAXD == 3) batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile);
+                break;
         }
     }
 }
@@ -1946,12 +1953,17 @@ int cpab_batch_fwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
     hipLaunchKernelGGL(k_batch_A, dim3((maxDt + BLOCK - 1) / BLOCK, n_inst), dim3(BLOCK), maxd * sizeof(float), stream,
                        dev_inst);
     hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, stream, counter);
-    if (save)
-        hipLaunchKernelGGL(k_batch_fwd<true>, dim3(blocks), dim3(BLOCK), lds_tab, stream, dev_inst, dev_item_inst,
-                           n_items, ppt, t_end, counter);
-    else
-        hipLaunchKernelGGL(k_batch_fwd<false>, dim3(blocks), dim3(BLOCK), lds_tab, stream, dev_inst, dev_item_inst,
-                           n_items, ppt, t_end, counter);
+    int maxnd = 1;
+    for (int i = 0; i < n_inst; ++i) maxnd = host_inst[i].tess.ndim > maxnd ? host_inst[i].tess.ndim : maxnd;
+#define LAUNCH_BF(S, M)                                                                                           \
+    hipLaunchKernelGGL((k_batch_fwd<S, M>), dim3(blocks), dim3(BLOCK), lds_tab, stream, dev_inst, dev_item_inst, \
+                       n_items, ppt, t_end, counter)
+    if (save) {
+        if (maxnd == 3) LAUNCH_BF(true, 3); else LAUNCH_BF(true, 2);
+    } else {
+        if (maxnd == 3) LAUNCH_BF(false, 3); else LAUNCH_BF(false, 2);
+    }
+#undef LAUNCH_BF
     return status_from_hip();
 }
 
@@ -1980,12 +1992,17 @@ int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
     const int box_cap = (have && want_box && lds_tab + lds_acc + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT) ? BOX_FLOATS : 0;
     const int box_off = (int)((lds_tab + lds_acc) / 4);
     const size_t lds = lds_tab + lds_acc + (size_t)box_cap * 4;
-    if (have)
-        hipLaunchKernelGGL(k_batch_bwd<true>, dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst, n_items,
-                           ppt, t_end, counter, acc_off, box_off, box_cap);
-    else
-        hipLaunchKernelGGL(k_batch_bwd<false>, dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst,
-                           n_items, ppt, t_end, counter, acc_off, box_off, 0);
+    int maxnd = 1;
+    for (int i = 0; i < n_inst; ++i) maxnd = host_inst[i].tess.ndim > maxnd ? host_inst[i].tess.ndim : maxnd;
+#define LAUNCH_BB(S, M, CAP)                                                                                     \
+    hipLaunchKernelGGL((k_batch_bwd<S, M>), dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst,    \
+                       n_items, ppt, t_end, counter, acc_off, box_off, CAP)
+    if (have) {
+        if (maxnd == 3) LAUNCH_BB(true, 3, box_cap); else LAUNCH_BB(true, 2, box_cap);
+    } else {
+        if (maxnd == 3) LAUNCH_BB(false, 3, 0); else LAUNCH_BB(false, 2, 0);
+    }
+#undef LAUNCH_BB
     if ((rc = status_from_hip())) return rc;
     hipLaunchKernelGGL(k_batch_reduce, dim3(n_inst), dim3(BLOCK), maxDt * sizeof(float), stream, dev_inst);
     return status_from_hip();

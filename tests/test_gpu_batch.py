@@ -20,10 +20,10 @@ PROBLEMS = [  # tess, zero_boundary, channels, data size, output size
 ]
 
 
-def _make(cuda, seed=0, save=True):
+def _make(cuda, seed=0, save=True, problems=PROBLEMS):
     g = torch.Generator().manual_seed(seed)
-    cpabs = [Cpab(nc, zero_boundary=zb) for nc, zb, _, _, _ in PROBLEMS]
-    probs = [(T, c, s, o) for T, (_, _, c, s, o) in zip(cpabs, PROBLEMS)]
+    cpabs = [Cpab(nc, zero_boundary=zb) for nc, zb, _, _, _ in problems]
+    probs = [(T, c, s, o) for T, (_, _, c, s, o) in zip(cpabs, problems)]
     b = CpabBatch(probs, device=cuda, save=save)
     for i, (T, c, s, o) in enumerate(probs):
         b.theta_views[i].copy_(torch.randn(T.get_theta_dim(), generator=g) * 0.4)
@@ -91,3 +91,15 @@ def test_autotune_and_graph_replays(cuda):
         # eager work between replays
         x = torch.randn(1000, device=cuda).sum()
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("save", [True, False])
+def test_batch_without_3d_instances_matches_single_problem_path(cuda, save):
+    """A batch of 1D/2D instances runs the batch kernels built without the 3D
+    path (fewer registers); same results as the single-problem path."""
+    b, probs = _make(cuda, save=save, problems=[p for p in PROBLEMS if len(p[0]) < 3])
+    b.set_launch(2, 512)
+    b.forward()
+    b.backward()
+    torch.cuda.synchronize()
+    _check(b, _reference(b, probs))
