@@ -945,17 +945,17 @@ __global__ __launch_bounds__(BLOCK) void k_interp_bwd(int N, int nP, Src s, Img 
 // theta (memory-bound: many waves keep loads in flight); every sum runs in a
 // fixed order.
 constexpr int RBLOCK = 1024;
-__global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
-                                                         int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
-                                                         float* __restrict__ gtheta, float* __restrict__ gA) {
-    extern __shared__ float sg[];  // [nw + 1, Dt]
-    const int th = blockIdx.x;
+// dL/dtheta of one theta from its tile partials pp [tiles, Dt] (q-space
+// dL/dA): the block's waves sum the tiles, the wave sums are added in wave
+// order, scaled to x-space (gA, nullable, receives dL/dA), and B^T dL/dA is
+// split over the waves by rows of B and added in wave order.  sg holds
+// [nw + 1, Dt] floats.
+__device__ __forceinline__ void reduce_theta(float* sg, const float* __restrict__ pp, int tiles, int Dt, int ndim,
+                                             const int (&nc)[3], const float* __restrict__ B, int d,
+                                             float* __restrict__ gth, float* __restrict__ gA) {
     const int nw = blockDim.x >> 6;
     const int P = ndim * (ndim + 1);
-    const int nc[3] = {nc0, nc1, nc2};
-    const float* pp = part + (size_t)th * tiles * Dt;
-    // wave w sums tiles w, w + nw, ... (two chains for overlap); the wave sums
-    // are then added in wave order
+    // wave w sums tiles w, w + nw, ... (two chains for overlap)
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = lane; i < Dt; i += 64) {
         float s0 = 0.f, s1 = 0.f;
@@ -976,7 +976,7 @@ __global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict
         const float sc = (col < ndim) ? (float)nc[r] / (float)nc[col] : (float)nc[r];
         s *= sc;
         sA[i] = s;
-        if (gA) gA[(size_t)th * Dt + i] = s;
+        if (gA) gA[i] = s;
     }
     __syncthreads();
     // B^T dL/dA: wave w takes rows [i0, i1) of B, the wave partials (reusing
@@ -991,8 +991,18 @@ __global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict
     for (int k = threadIdx.x; k < d; k += blockDim.x) {
         float s = sg[k];
         for (int w2 = 1; w2 < nw; ++w2) s += sg[w2 * d + k];
-        gtheta[(size_t)th * d + k] = s;
+        gth[k] = s;
     }
+}
+
+__global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
+                                                         int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
+                                                         float* __restrict__ gtheta, float* __restrict__ gA) {
+    extern __shared__ float sg[];  // [nw + 1, Dt]
+    const int th = blockIdx.x;
+    const int nc[3] = {nc0, nc1, nc2};
+    reduce_theta(sg, part + (size_t)th * tiles * Dt, tiles, Dt, ndim, nc, B, d, gtheta + (size_t)th * d,
+                 gA ? gA + (size_t)th * Dt : nullptr);
 }
 
 // cell index of each point (libcpab/numpy/findcellidx.py semantics, FP32)
@@ -1200,26 +1210,14 @@ __global__ __launch_bounds__(BLOCK, MAXD == 2 ? 4 : 1) void k_batch_bwd(const cp
 }
 
 // per instance: dL/dtheta = B^T (sum over its items of the partials, in x-space)
-__global__ __launch_bounds__(BLOCK) void k_batch_reduce(const cpab_instance_t* __restrict__ inst) {
-    extern __shared__ float sg[];
+__global__ __launch_bounds__(RBLOCK) void k_batch_reduce(const cpab_instance_t* __restrict__ inst) {
+    extern __shared__ float sg[];  // [nw + 1, largest Dt of the batch]
     const cpab_instance_t* I = inst + blockIdx.x;
-    const int nd = I->tess.ndim, d = I->tess.d;
-    const int P = nd * (nd + 1);
-    const int Dt = I->tess.nC * P;
+    const int nd = I->tess.ndim;
+    const int Dt = I->tess.nC * nd * (nd + 1);
     const float* part = reinterpret_cast<const float*>(static_cast<const char*>(I->ws) + align256_hd((size_t)Dt * 4));
-    for (int i = threadIdx.x; i < Dt; i += BLOCK) {
-        float s = 0.f;
-        for (int t = 0; t < I->tiles; ++t) s += part[(size_t)t * Dt + i];
-        const int rem = i % P, r = rem / (nd + 1), col = rem % (nd + 1);
-        const float sc = (col < nd) ? (float)I->tess.nc[r] / (float)I->tess.nc[col] : (float)I->tess.nc[r];
-        sg[i] = s * sc;
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < d; k += BLOCK) {
-        float s = 0.f;
-        for (int i = 0; i < Dt; ++i) s = fmaf(I->tess.B[(size_t)i * d + k], sg[i], s);
-        I->grad_theta[k] = s;
-    }
+    const int nc[3] = {I->tess.nc[0], I->tess.nc[1], I->tess.nc[2]};
+    reduce_theta(sg, part, I->tiles, Dt, nd, nc, I->tess.B, I->tess.d, I->grad_theta, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -2004,7 +2002,12 @@ int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
     }
 #undef LAUNCH_BB
     if ((rc = status_from_hip())) return rc;
-    hipLaunchKernelGGL(k_batch_reduce, dim3(n_inst), dim3(BLOCK), maxDt * sizeof(float), stream, dev_inst);
+    // as many waves (up to 16) as the LDS holds [nw + 1, maxDt] floats for
+    int nw = RBLOCK / 64;
+    while (nw > 1 && (size_t)(nw + 1) * maxDt * sizeof(float) > LDS_LIMIT) nw >>= 1;
+    if ((size_t)(nw + 1) * maxDt * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
+    hipLaunchKernelGGL(k_batch_reduce, dim3(n_inst), dim3(nw * 64), (nw + 1) * maxDt * sizeof(float), stream,
+                       dev_inst);
     return status_from_hip();
 }
 
