@@ -1081,8 +1081,13 @@ __device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, flo
     im.s2 = nd > 2 ? I->dsize[2] : 1;
 }
 
-__device__ __forceinline__ int next_item(int* counter, int* s_item) {
-    if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1);
+// A block takes GRAB consecutive work items per atomic: consecutive items
+// mostly belong to one instance, whose cell tables are then staged once for
+// all of them.
+constexpr int GRAB = 4;
+
+__device__ __forceinline__ int next_grab(int* counter, int* s_item) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(counter, GRAB);
     __syncthreads();
     const int item = __builtin_amdgcn_readfirstlane(*s_item);
     __syncthreads();
@@ -1128,34 +1133,39 @@ __global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __re
     __shared__ int s_item;
     int cur = -1;
     for (;;) {
-        const int item = next_item(counter, &s_item);
-        if (item >= n_items) break;
-        const int ii = __builtin_amdgcn_readfirstlane(item_inst[item]);
-        const cpab_instance_t* I = inst + ii;
-        Launch L;
-        Geo g;
-        Src s;
-        Img im;
-        item_args(I, ppt, t_end, L, g, s, im);
-        const int tile = item - I->item0;
-        const bool restage = ii != cur;
-        cur = ii;
-        const float* Aq = static_cast<const float*>(I->ws);
-        switch (g.ndim) {
-            case 1: {
-                const CellTable<1> tab = restage ? stage_tables<1>(smem, Aq, g) : table_view<1>(smem, g.nC);
-                fused_fwd_tile<1, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
-            } break;
-            case 2: {
-                const CellTable<2> tab = restage ? stage_tables<2>(smem, Aq, g) : table_view<2>(smem, g.nC);
-                fused_fwd_tile<2, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
-            } break;
-            default:
-                if constexpr (MAXD == 3) {
-                    const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC);
-                    fused_fwd_tile<3, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
-                }
-                break;
+        const int first = next_grab(counter, &s_item);
+        if (first >= n_items) break;
+        const int last = first + GRAB < n_items ? first + GRAB : n_items;
+        for (int item = first; item < last; ++item) {
+            // the previous item's waves are done with the tables and accumulators
+            if (item != first) __syncthreads();
+            const int ii = __builtin_amdgcn_readfirstlane(item_inst[item]);
+            const cpab_instance_t* I = inst + ii;
+            Launch L;
+            Geo g;
+            Src s;
+            Img im;
+            item_args(I, ppt, t_end, L, g, s, im);
+            const int tile = item - I->item0;
+            const bool restage = ii != cur;
+            cur = ii;
+            const float* Aq = static_cast<const float*>(I->ws);
+            switch (g.ndim) {
+                case 1: {
+                    const CellTable<1> tab = restage ? stage_tables<1>(smem, Aq, g) : table_view<1>(smem, g.nC);
+                    fused_fwd_tile<1, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+                } break;
+                case 2: {
+                    const CellTable<2> tab = restage ? stage_tables<2>(smem, Aq, g) : table_view<2>(smem, g.nC);
+                    fused_fwd_tile<2, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+                } break;
+                default:
+                    if constexpr (MAXD == 3) {
+                        const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC);
+                        fused_fwd_tile<3, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
+                    }
+                    break;
+            }
         }
     }
 }
@@ -1185,26 +1195,32 @@ __global__ __launch_bounds__(BLOCK, MAXD == 2 ? 4 : 1) void k_batch_bwd(const cp
     __shared__ int s_item;
     int cur = -1;
     for (;;) {
-        const int item = next_item(counter, &s_item);
-        if (item >= n_items) break;
-        const int ii = __builtin_amdgcn_readfirstlane(item_inst[item]);
-        const cpab_instance_t* I = inst + ii;
-        Launch L;
-        Geo g;
-        Src s;
-        Img im;
-        item_args(I, ppt, t_end, L, g, s, im);
-        const int tile = item - I->item0;
-        const bool restage = ii != cur;
-        cur = ii;
-        float* acc = smem + acc_offset;  // after the largest table of the batch
-        float* box = box_cap ? smem + box_offset : nullptr;
-        switch (g.ndim) {
-            case 1: batch_bwd_item<1, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
-            case 2: batch_bwd_item<2, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
-            default:
-                if constexpr (MAXD == 3) batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile);
-                break;
+        const int first = next_grab(counter, &s_item);
+        if (first >= n_items) break;
+        const int last = first + GRAB < n_items ? first + GRAB : n_items;
+        for (int item = first; item < last; ++item) {
+            // the previous item's waves are done with the tables and accumulators
+            if (item != first) __syncthreads();
+            const int ii = __builtin_amdgcn_readfirstlane(item_inst[item]);
+            const cpab_instance_t* I = inst + ii;
+            Launch L;
+            Geo g;
+            Src s;
+            Img im;
+            item_args(I, ppt, t_end, L, g, s, im);
+            const int tile = item - I->item0;
+            const bool restage = ii != cur;
+            cur = ii;
+            float* acc = smem + acc_offset;  // after the largest table of the batch
+            float* box = box_cap ? smem + box_offset : nullptr;
+            switch (g.ndim) {
+                case 1: batch_bwd_item<1, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
+                case 2: batch_bwd_item<2, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
+                default:
+                    if constexpr (MAXD == 3)
+                        batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile);
+                    break;
+            }
         }
     }
 }
