@@ -80,6 +80,24 @@ def test_argument_errors_are_status_codes():
     assert L.cpab_interpolate_fwd(2, 1, 1, 1, dsz, 1, osz, 1, None) == 1
 
 
+def test_tables_beyond_the_lds_budget_are_an_error_before_any_launch():
+    """A tessellation whose cell tables do not fit the 160 KB of LDS comes back
+    as CPAB_ERR_LDS (5) from the argument checks, before anything is
+    launched (the pointers below are never dereferenced)."""
+    L = N.lib()
+    t = N.CpabTess()
+    t.ndim = 2
+    t.nc = N.int3([40, 40, 1])
+    t.nC = 4 * 40 * 40
+    t.d = 10
+    t.B = 1
+    t.planes = 1
+    t.nbr = 1
+    need = L.cpab_workspace_bytes(ctypes.byref(t), 1, 100)
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 1, 1, 100, 0, 1.0, 1, None, 1, need, None) == 5
+    assert b"LDS" in L.cpab_status_string(5) or b"lds" in L.cpab_status_string(5).lower()
+
+
 def test_workspace_and_saved_sizes():
     L = N.lib()
     t = _tess("2d_4x4")
