@@ -133,9 +133,12 @@ class Cpab(object):
 
     def set_solver_params(self, nstepsolver=50, numeric_grad=False, use_slow=False, method=None,
                           store_jacobian=None, launch=None):
-        """libcpab/cpab.py:140-157.  nstepsolver / numeric_grad / use_slow apply
-        to method='numeric' (the reference's fixed-step scheme); the default
-        method='closed_form' integrates exactly and ignores them."""
+        """libcpab/cpab.py:140-157.  nstepsolver and numeric_grad apply to
+        method='numeric' (the reference's fixed-step scheme); the default
+        method='closed_form' integrates exactly and ignores them.  use_slow
+        (the reference's pure-torch version of the same scheme,
+        libcpab/pytorch/transformer.py:72-128) is accepted and checked but
+        selects the same HIP kernels: there is no torch fallback path."""
         assert nstepsolver > 0, "nstepsolver must be a positive number"
         assert type(nstepsolver) == int, "nstepsolver must be integer"
         assert type(numeric_grad) == bool, "numeric_grad must be bool"
