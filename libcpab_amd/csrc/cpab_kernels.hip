@@ -10,7 +10,9 @@
 //   k_transform_bwd     adjoint of k_transform, dL/dA partials per tile (+ dL/dpoints)
 //   k_transform_data_bwd  adjoint of k_transform_data (+ dL/ddata scatter)
 //   k_interp_fwd / k_interp_bwd   linear interpolation of data at a grid
-//   k_reduce_theta      per-theta sum of the tile partials, then B^T (deterministic)
+//   k_reduce_theta      per-theta sum of the tile partials, then B^T (many thetas)
+//   k_reduce_cols       column slices of that sum (few thetas), then
+//   k_reduce_bt         B^T per theta (all in a fixed order: deterministic)
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stddef.h>
@@ -984,6 +986,7 @@ __device__ __forceinline__ void reduce_theta(float* sg, const float* __restrict_
     const int i0 = (int)((long long)Dt * w / nw), i1 = (int)((long long)Dt * (w + 1) / nw);
     for (int k = lane; k < d; k += 64) {
         float s = 0.f;
+#pragma unroll 8
         for (int i = i0; i < i1; ++i) s = fmaf(B[(size_t)i * d + k], sA[i], s);
         sg[w * d + k] = s;
     }
@@ -995,6 +998,8 @@ __device__ __forceinline__ void reduce_theta(float* sg, const float* __restrict_
     }
 }
 
+// Both steps in one block per theta, for launches with enough thetas to fill
+// the chip by themselves.
 __global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict__ part, int tiles, int Dt, int ndim,
                                                          int nc0, int nc1, int nc2, const float* __restrict__ B, int d,
                                                          float* __restrict__ gtheta, float* __restrict__ gA) {
@@ -1003,6 +1008,72 @@ __global__ __launch_bounds__(RBLOCK) void k_reduce_theta(const float* __restrict
     const int nc[3] = {nc0, nc1, nc2};
     reduce_theta(sg, part + (size_t)th * tiles * Dt, tiles, Dt, ndim, nc, B, d, gtheta + (size_t)th * d,
                  gA ? gA + (size_t)th * Dt : nullptr);
+}
+
+// Column slice blockIdx.y (columns [c0, c0 + cw)) of theta blockIdx.x's
+// dL/dA: the block's waves sum the tiles (wave w: tiles w, w + nw, ...), the
+// wave sums are added in wave order and scaled to x-space -- per column the
+// same arithmetic as reduce_theta.  The result overwrites the tile-0 row of
+// the partials (each block owns its columns and has read them all first) and
+// goes to gA when given.  Slicing the columns gives a few thetas enough
+// blocks to fill the chip.
+__global__ __launch_bounds__(RBLOCK) void k_reduce_cols(float* __restrict__ part, int tiles, int Dt, int cw, int ndim,
+                                                        int nc0, int nc1, int nc2, float* __restrict__ gA) {
+    extern __shared__ float sg[];  // [nw, cw]
+    const int th = blockIdx.x;
+    const int c0 = blockIdx.y * cw, c1 = (c0 + cw < Dt) ? c0 + cw : Dt;
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int P = ndim * (ndim + 1);
+    const int nc[3] = {nc0, nc1, nc2};
+    float* pp = part + (size_t)th * tiles * Dt;
+    for (int i = c0 + lane; i < c1; i += 64) {
+        float s0 = 0.f, s1 = 0.f;
+        int t = w;
+        for (; t + nw < tiles; t += 2 * nw) {
+            s0 += pp[(size_t)t * Dt + i];
+            s1 += pp[(size_t)(t + nw) * Dt + i];
+        }
+        if (t < tiles) s0 += pp[(size_t)t * Dt + i];
+        sg[w * cw + (i - c0)] = s0 + s1;
+    }
+    __syncthreads();
+    for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+        float s = sg[i - c0];
+        for (int k = 1; k < nw; ++k) s += sg[k * cw + (i - c0)];
+        const int rem = i % P, r = rem / (ndim + 1), col = rem % (ndim + 1);
+        const float sc = (col < ndim) ? (float)nc[r] / (float)nc[col] : (float)nc[r];
+        s *= sc;
+        pp[i] = s;
+        if (gA) gA[(size_t)th * Dt + i] = s;
+    }
+}
+
+// dL/dtheta = B^T dL/dA for theta blockIdx.x, dL/dA (x-space) in the tile-0
+// row of its partials: wave w takes rows [i0, i1) of B, the wave partials are
+// added in wave order.
+__global__ __launch_bounds__(RBLOCK) void k_reduce_bt(const float* __restrict__ part, int tiles, int Dt,
+                                                      const float* __restrict__ B, int d, float* __restrict__ gtheta) {
+    extern __shared__ float sg[];  // [Dt + nw * d]
+    const int th = blockIdx.x;
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float* pp = part + (size_t)th * tiles * Dt;
+    float* sA = sg;
+    float* sw = sg + Dt;
+    for (int i = threadIdx.x; i < Dt; i += blockDim.x) sA[i] = pp[i];
+    __syncthreads();
+    const int i0 = (int)((long long)Dt * w / nw), i1 = (int)((long long)Dt * (w + 1) / nw);
+    for (int k = lane; k < d; k += 64) {
+        float s = 0.f;
+#pragma unroll 8
+        for (int i = i0; i < i1; ++i) s = fmaf(B[(size_t)i * d + k], sA[i], s);
+        sw[w * d + k] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < d; k += blockDim.x) {
+        float s = sw[k];
+        for (int w2 = 1; w2 < nw; ++w2) s += sw[w2 * d + k];
+        gtheta[(size_t)th * d + k] = s;
+    }
 }
 
 // cell index of each point (libcpab/numpy/findcellidx.py semantics, FP32)
@@ -1478,16 +1549,39 @@ int launch_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, flo
     return status_from_hip();
 }
 
-int launch_reduce(const cpab_tess_t* t, const float* part, int n_theta, int tiles, float* gtheta, float* gA,
+// dL/dtheta from the per-tile partials (overwritten): column-sliced tile sums,
+// then B^T per theta.  Deterministic: fixed summation order throughout.
+int launch_reduce(const cpab_tess_t* t, float* part, int n_theta, int tiles, float* gtheta, float* gA,
                   hipStream_t st) {
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
     Geo g = make_geo(t);
-    // as many waves (up to 16) as the LDS holds [nw + 1, Dt] floats for
-    int nw = RBLOCK / 64;
-    while (nw > 1 && (size_t)(nw + 1) * Dt * sizeof(float) > LDS_LIMIT) nw >>= 1;
-    if ((size_t)(nw + 1) * Dt * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
-    hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(nw * 64), (nw + 1) * Dt * sizeof(float), st, part, tiles,
-                       Dt, t->ndim, g.nc0, g.nc1, g.nc2, t->B, t->d, gtheta, gA);
+    // column slices of at least 64 columns, enough of them for ~1024 blocks
+    int nslice = (1024 + n_theta - 1) / n_theta;
+    const int max_slices = (Dt + 63) / 64;
+    if (nslice > max_slices) nslice = max_slices;
+    if (nslice < 1) nslice = 1;
+    if (nslice == 1) {
+        // enough thetas: one block per theta, as many waves (up to 16) as the
+        // LDS holds [nw + 1, Dt] floats for
+        int nw = RBLOCK / 64;
+        while (nw > 1 && (size_t)(nw + 1) * Dt * sizeof(float) > LDS_LIMIT) nw >>= 1;
+        if ((size_t)(nw + 1) * Dt * sizeof(float) <= LDS_LIMIT) {
+            hipLaunchKernelGGL(k_reduce_theta, dim3(n_theta), dim3(nw * 64), (nw + 1) * Dt * sizeof(float), st, part,
+                               tiles, Dt, t->ndim, g.nc0, g.nc1, g.nc2, t->B, t->d, gtheta, gA);
+            return status_from_hip();
+        }
+    }
+    const int cw = (Dt + nslice - 1) / nslice;
+    nslice = (Dt + cw - 1) / cw;
+    const int nw = RBLOCK / 64;
+    hipLaunchKernelGGL(k_reduce_cols, dim3(n_theta, nslice), dim3(nw * 64), (size_t)nw * cw * sizeof(float), st,
+                       part, tiles, Dt, cw, t->ndim, g.nc0, g.nc1, g.nc2, gA);
+    // B^T: as many waves (up to 16) as the LDS holds Dt + nw * d floats for
+    int nwb = RBLOCK / 64;
+    while (nwb > 1 && (size_t)(Dt + nwb * t->d) * sizeof(float) > LDS_LIMIT) nwb >>= 1;
+    if ((size_t)(Dt + nwb * t->d) * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
+    hipLaunchKernelGGL(k_reduce_bt, dim3(n_theta), dim3(nwb * 64), (size_t)(Dt + nwb * t->d) * sizeof(float), st,
+                       part, tiles, Dt, t->B, t->d, gtheta);
     return status_from_hip();
 }
 
