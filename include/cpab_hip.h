@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CPAB_ABI_VERSION 3
+#define CPAB_ABI_VERSION 4
 
 #define CPAB_OK 0
 #define CPAB_ERR_ARG 1
@@ -82,14 +82,19 @@ size_t cpab_jac_floats(int ndim, int n_theta, int nP);
 int cpab_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, float* Aq, hipStream_t stream);
 
 /* Closed-form integration of the points for time t_end (1 in the Cpab API).
+ * grid_shape (nullable, host int[2]): the 2D points form a meshgrid of
+ * grid_shape[0] x grid_shape[1] points, x fastest (as from
+ * libcpab/pytorch/functions.py:102-108); they are then walked in compact 2D
+ * tiles, which changes no point's result.  Pass the same value to
+ * cpab_transform_bwd.
  * jac (nullable, cpab_jac_floats) receives per theta the q-space Jacobian
  * [ndim*ndim, nP] and (2D/3D) the chunk log [3, nP], which let
  * cpab_transform_bwd skip its first pass and replay the chunks of its second.  Replaces cpab_gpu.forward,
  * libcpab/pytorch/transformer_cuda.cpp:17-41 (+ the Trels expm of
  * libcpab/pytorch/transformer.py:152-155). */
 int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
-                       int per_theta_grid, float t_end, float* out, float* jac, void* workspace, size_t ws_bytes,
-                       hipStream_t stream);
+                       int per_theta_grid, const int* grid_shape, float t_end, float* out, float* jac,
+                       void* workspace, size_t ws_bytes, hipStream_t stream);
 
 /* grad_theta [n_theta, d] = d<grad_out, transform(points)>/d theta, and (if
  * grad_points != NULL) its gradient w.r.t. the points [n_theta, ndim, nP].
@@ -97,8 +102,8 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
  * the host-side reduction of libcpab/pytorch/transformer.py:201; the
  * reference returns no point gradient (transformer.py:202). */
 int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
-                       int per_theta_grid, float t_end, const float* grad_out, const float* jac, float* grad_theta,
-                       float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream);
+                       int per_theta_grid, const int* grid_shape, float t_end, const float* grad_out, const float* jac,
+                       float* grad_theta, float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream);
 
 /* Validation: the cell sequence of each point's trajectory under the same
  * integration as cpab_transform_fwd, as segments of positive duration

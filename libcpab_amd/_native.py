@@ -39,10 +39,10 @@ _SIGNATURES = {
     "cpab_saved_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "cpab_jac_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "cpab_theta_to_A": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_ptr]),
-    "cpab_transform_fwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
-                                    _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
-    "cpab_transform_bwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
-                                    _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_fwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_ptr,
+                                    _c_float, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_bwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_ptr,
+                                    _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "cpab_transform_cells": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_float,
                                       _c_int, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "cpab_interpolate_fwd": (_c_int, [_c_int, _c_ptr, _c_int, _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr]),

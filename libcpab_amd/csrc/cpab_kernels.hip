@@ -426,8 +426,8 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
     const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
     for (int r = 0; r < L.ppt; ++r) {
-        const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
-        if (p >= L.nP) break;
+        const int p = tile_t(L, s, tile, r);  // point index (2D meshgrid tiles when L.ty > 0)
+        if (p < 0) continue;
         float x[D], q[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -613,8 +613,8 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
     const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
     for (int r = 0; r < L.ppt; ++r) {
-        const int p = (tile * L.ppt + r) * BLOCK + threadIdx.x;
-        const bool valid = p < L.nP;
+        const int p = tile_t(L, s, tile, r);  // point index (2D meshgrid tiles when L.ty > 0)
+        const bool valid = p >= 0;
         float q0[D], mu[D];
         int c0 = 0;
         ChunkLog lg;
@@ -1607,6 +1607,31 @@ const char* cpab_status_string(int status) {
 
 int cpab_abi_version(void) { return CPAB_ABI_VERSION; }
 
+// Launch shape of a grid transform.  2D points that form a meshgrid of
+// shape[0] x shape[1] (x fastest, shape[0] * shape[1] == nP) run in 2D tiles --
+// tile_t with the axes swapped, so its index i1 * shape[0] + i0 is the point
+// index -- when at most max_tiles tiles result (the partials the caller's
+// workspace holds; 0: no limit); everything else in runs of consecutive points.
+// The order changes which points share a tile, not any point's result.
+static void grid_launch(const cpab_tess_t* t, int n_theta, int nP, const int* shape, size_t max_tiles, Launch& L,
+                        Src& s) {
+    L.n_theta = n_theta;
+    L.nP = nP;
+    L.ppt = pick_ppt(n_theta, nP);
+    L.ty = 0;
+    L.tiles = nP > 0 ? (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt) : 1;
+    if (shape && t->ndim == 2 && shape[0] > 0 && shape[1] > 0 && (long long)shape[0] * shape[1] == nP) {
+        const int ty = tile_ty(2, shape[0]);
+        const int tiles = tile_count(2, nP, shape[1], shape[0], L.ppt);
+        if (ty > 0 && (max_tiles == 0 || (size_t)tiles <= max_tiles)) {
+            L.ty = ty;
+            L.tiles = tiles;
+            s.g0 = shape[1];
+            s.g1 = shape[0];
+        }
+    }
+}
+
 size_t cpab_workspace_bytes(const cpab_tess_t* t, int n_theta, int nP) {
     if (check_tess(t) != CPAB_OK || n_theta < 0 || nP < 0) return 0;
     const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
@@ -1632,8 +1657,8 @@ int cpab_theta_to_A(const cpab_tess_t* t, const float* theta, int n_theta, float
     }
 
 int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
-                       int per_theta_grid, float t_end, float* out, float* jac, void* workspace, size_t ws_bytes,
-                       hipStream_t stream) {
+                       int per_theta_grid, const int* grid_shape, float t_end, float* out, float* jac,
+                       void* workspace, size_t ws_bytes, hipStream_t stream) {
     int rc = check_tess(t);
     if (rc) return rc;
     if (n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
@@ -1645,14 +1670,10 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     Launch L;
-    L.n_theta = n_theta;
-    L.nP = nP;
-    L.ppt = pick_ppt(n_theta, nP);
-    L.ty = 0;
-    L.tiles = (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt);
+    Src s = {};
+    grid_launch(t, n_theta, nP, grid_shape, 0, L, s);
     L.t_end = t_end;
     Geo g = make_geo(t);
-    Src s = {};
     s.pts = points;
     s.per_theta = per_theta_grid;
     dim3 grid(L.tiles * n_theta);
@@ -1699,8 +1720,8 @@ int cpab_transform_cells(const cpab_tess_t* t, const float* theta, int n_theta, 
 }
 
 int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* points, int nP,
-                       int per_theta_grid, float t_end, const float* grad_out, const float* jac, float* grad_theta,
-                       float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream) {
+                       int per_theta_grid, const int* grid_shape, float t_end, const float* grad_out, const float* jac,
+                       float* grad_theta, float* grad_points, void* workspace, size_t ws_bytes, hipStream_t stream) {
     int rc = check_tess(t);
     if (rc) return rc;
     if (n_theta < 0 || nP < 0) return CPAB_ERR_ARG;
@@ -1715,14 +1736,12 @@ int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) + align256((size_t)n_theta * Dt * 4));
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     Launch L;
-    L.n_theta = n_theta;
-    L.nP = nP;
-    L.ppt = pick_ppt(n_theta, nP);
-    L.ty = 0;
-    L.tiles = nP > 0 ? (nP + BLOCK * L.ppt - 1) / (BLOCK * L.ppt) : 1;
+    Src s = {};
+    // 2D tiles only as far as the workspace holds their partials
+    const size_t head = align256((size_t)n_theta * Dt * 4);
+    grid_launch(t, n_theta, nP, grid_shape, (ws_bytes - head) / ((size_t)n_theta * Dt * 4), L, s);
     L.t_end = t_end;
     Geo g = make_geo(t);
-    Src s = {};
     s.pts = points;
     s.per_theta = per_theta_grid;
     dim3 grid(L.tiles * n_theta);

@@ -95,7 +95,9 @@ def _int_arr(vals):
 # --------------------------------------------------------------------------
 class TransformGrid(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, points, theta, params, t_end, store_jac=None):
+    def forward(ctx, points, theta, params, t_end, store_jac=None, grid_shape=None):
+        """grid_shape (optional [n0, n1]): the 2D points are a meshgrid of that
+        shape in uniform_meshgrid order, walked in compact tiles."""
         _require_gpu_f32("grid", points)
         _require_gpu_f32("theta", theta)
         dev = device_tess(params, theta.device)
@@ -119,11 +121,13 @@ class TransformGrid(torch.autograd.Function):
             jac = torch.empty(int(N.lib().cpab_jac_floats(ndim, n_theta, nP)), device=theta.device,
                               dtype=torch.float32)
         ws, nb = dev.workspace(n_theta, nP)
-        N.check(N.lib().cpab_transform_fwd(dev.ref, _ptr(theta), n_theta, _ptr(points), nP, per_theta,
+        shape = _int_arr(grid_shape) if grid_shape is not None else None
+        N.check(N.lib().cpab_transform_fwd(dev.ref, _ptr(theta), n_theta, _ptr(points), nP, per_theta, shape,
                                            float(t_end), _ptr(out), _ptr(jac), _ptr(ws), nb, _stream(theta.device)))
         ctx.params = params
         ctx.t_end = t_end
         ctx.per_theta = per_theta
+        ctx.grid_shape = grid_shape
         ctx.save_for_backward(points, theta, jac)
         return out
 
@@ -140,12 +144,13 @@ class TransformGrid(torch.autograd.Function):
         if ctx.needs_input_grad[0]:
             gpts = torch.empty(n_theta, dev.ndim, nP, device=theta.device, dtype=torch.float32)
         ws, nb = dev.workspace(n_theta, nP)
-        N.check(N.lib().cpab_transform_bwd(dev.ref, _ptr(theta), n_theta, _ptr(points), nP, ctx.per_theta,
+        shape = _int_arr(ctx.grid_shape) if ctx.grid_shape is not None else None
+        N.check(N.lib().cpab_transform_bwd(dev.ref, _ptr(theta), n_theta, _ptr(points), nP, ctx.per_theta, shape,
                                            float(ctx.t_end), _ptr(grad), _ptr(jac), _ptr(gtheta), _ptr(gpts),
                                            _ptr(ws), nb, _stream(theta.device)))
         if gpts is not None and not ctx.per_theta:
             gpts = gpts.sum(0)
-        return gpts, (gtheta if ctx.needs_input_grad[1] else None), None, None, None
+        return gpts, (gtheta if ctx.needs_input_grad[1] else None), None, None, None, None
 
 
 # --------------------------------------------------------------------------

@@ -49,7 +49,7 @@ class CpabSequential(object):
     def identity(self, n_sample, epsilon=0):
         return [c.identity(n_sample, epsilon) for c in self.cpab]
 
-    def _stage(self, i, grid, theta):
+    def _stage(self, i, grid, theta, grid_shape=None):
         c = self.cpab[i]
         c._check_type(grid)
         c._check_device(grid)
@@ -57,23 +57,25 @@ class CpabSequential(object):
         c._check_device(theta)
         if getattr(c.params, "method", "closed_form") != "closed_form":
             return c.transform_grid(grid, theta)
-        return TransformGrid.apply(grid, theta, c.params, 1.0, False)
+        return TransformGrid.apply(grid, theta, c.params, 1.0, False, grid_shape)
 
-    def transform_grid(self, grid, thetas, output_all=False):
+    def transform_grid(self, grid, thetas, output_all=False, _grid_shape=None):
         self._assert_theta_shape(thetas)
         if not output_all:
             for i in range(self.n_cpab):
-                grid = self._stage(i, grid, thetas[i])
+                grid = self._stage(i, grid, thetas[i], _grid_shape)
             return grid
-        grids = [self._stage(0, grid, thetas[0])]
+        grids = [self._stage(0, grid, thetas[0], _grid_shape)]
         for i in range(1, self.n_cpab):
-            grids.append(self._stage(i, grids[-1], thetas[i]))
+            grids.append(self._stage(i, grids[-1], thetas[i], _grid_shape))
         return grids
 
     def transform_data(self, data, thetas, outsize, output_all=False):
         self._assert_theta_shape(thetas)
         grid = self.uniform_meshgrid(outsize)
-        grid_t = self.transform_grid(grid, thetas, output_all=output_all)
+        # every stage's points stay in meshgrid order: 2D stages walk them in tiles
+        shape = list(outsize) if self.ndim == 2 else None
+        grid_t = self.transform_grid(grid, thetas, output_all=output_all, _grid_shape=shape)
         if not output_all:
             return self.cpab[-1].interpolate(data, grid_t, outsize)
         return [self.cpab[i].interpolate(data, grid_t[i], outsize) for i in range(self.n_cpab)]

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = N.lib()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.cpab_abi_version() == 3
+    assert lib.cpab_abi_version() == 4
     assert lib.cpab_status_string(0) == b"ok"
     assert b"workspace" in lib.cpab_status_string(3)
 
@@ -55,25 +55,25 @@ def test_argument_errors_are_status_codes():
     # bad ndim
     bad = _tess()
     bad.ndim = 4
-    assert L.cpab_transform_fwd(ctypes.byref(bad), 1, 1, 1, 10, 0, 1.0, 1, None, 1, 1 << 20, None) == 2
+    assert L.cpab_transform_fwd(ctypes.byref(bad), 1, 1, 1, 10, 0, None, 1.0, 1, None, 1, 1 << 20, None) == 2
     # inconsistent cell count
     bad = _tess()
     bad.nC = 7
-    assert L.cpab_transform_fwd(ctypes.byref(bad), 1, 1, 1, 10, 0, 1.0, 1, None, 1, 1 << 20, None) == 1
+    assert L.cpab_transform_fwd(ctypes.byref(bad), 1, 1, 1, 10, 0, None, 1.0, 1, None, 1, 1 << 20, None) == 1
     # missing output
-    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 1, 1, 10, 0, 1.0, None, None, 1, 1 << 20, None) == 1
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 1, 1, 10, 0, None, 1.0, None, None, 1, 1 << 20, None) == 1
     # workspace too small
     need = L.cpab_workspace_bytes(ctypes.byref(t), 4, 1000)
     assert need > 0
-    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 4, 1, 1000, 0, 1.0, 1, None, 1, need - 1, None) == 3
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 4, 1, 1000, 0, None, 1.0, 1, None, 1, need - 1, None) == 3
     # empty problems are a no-op
-    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 0, 1, 1000, 0, 1.0, 1, None, None, 0, None) == 0
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 0, 1, 1000, 0, None, 1.0, 1, None, None, 0, None) == 0
     # ... and their empty arrays may be NULL
-    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 2, None, 0, 0, 1.0, None, None, None, 0, None) == 0
-    assert L.cpab_transform_fwd(ctypes.byref(t), None, 0, None, 0, 0, 1.0, None, None, None, 0, None) == 0
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 2, None, 0, 0, None, 1.0, None, None, None, 0, None) == 0
+    assert L.cpab_transform_fwd(ctypes.byref(t), None, 0, None, 0, 0, None, 1.0, None, None, None, 0, None) == 0
     assert L.cpab_findcell(2, (ctypes.c_int * 2)(3, 3), None, 0, None, None) == 0
     # a negative size is an error before any pointer is looked at
-    assert L.cpab_transform_fwd(ctypes.byref(t), None, 2, None, -1, 0, 1.0, None, None, None, 0, None) == 1
+    assert L.cpab_transform_fwd(ctypes.byref(t), None, 2, None, -1, 0, None, 1.0, None, None, None, 0, None) == 1
     assert L.cpab_interpolate_fwd(5, 1, 1, 1, None, 1, None, 1, None) == 2
     dsz = (ctypes.c_int * 3)(4, 4, 1)
     osz = (ctypes.c_int * 3)(0, 4, 1)
@@ -94,7 +94,7 @@ def test_tables_beyond_the_lds_budget_are_an_error_before_any_launch():
     t.planes = 1
     t.nbr = 1
     need = L.cpab_workspace_bytes(ctypes.byref(t), 1, 100)
-    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 1, 1, 100, 0, 1.0, 1, None, 1, need, None) == 5
+    assert L.cpab_transform_fwd(ctypes.byref(t), 1, 1, 1, 100, 0, None, 1.0, 1, None, 1, need, None) == 5
     assert b"LDS" in L.cpab_status_string(5) or b"lds" in L.cpab_status_string(5).lower()
 
 
