@@ -897,13 +897,40 @@ __global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, S
 }
 
 // interpolation at an explicit grid [N, D, nP]; out [N, C, outsize], output order
+// The (instance n, output index t, grid index p) of this thread.  2D outputs
+// run in 16 x 16 blocks of 8 x 8 wave tiles, so the output accesses (last
+// axis fastest) and the grid accesses (first axis fastest) both come in runs
+// of 8 lanes; blocks: N * ceil(g0 / 16) * ceil(g1 / 16).  Other
+// dimensions run in output order.
+template <int D>
+__device__ __forceinline__ bool interp_item(const Src& s, int N, int nP, int& n, int& t, int& p) {
+    if constexpr (D == 2) {
+        const int t0 = (s.g0 + 15) >> 4, t1 = (s.g1 + 15) >> 4;
+        const int b = blockIdx.x;
+        n = b / (t0 * t1);
+        const int rem = b - n * t0 * t1;
+        const int tid = threadIdx.x;
+        const int i0 = (rem / t1) * 16 + ((tid & 63) >> 3) + ((tid >> 7) << 3);
+        const int i1 = (rem % t1) * 16 + (tid & 7) + (((tid >> 6) & 1) << 3);
+        if (n >= N || i0 >= s.g0 || i1 >= s.g1) return false;
+        t = i0 * s.g1 + i1;
+        p = i0 + s.g0 * i1;
+        return true;
+    } else {
+        const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+        if (gid >= (size_t)N * nP) return false;
+        n = (int)(gid / nP);
+        t = (int)(gid % nP);
+        int ix[3];
+        p = out_to_point<D>(s, t, ix);
+        return true;
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_interp_fwd(int N, int nP, Src s, Img im, float* __restrict__ out) {
-    const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (gid >= (size_t)N * nP) return;
-    const int n = gid / nP, t = gid % nP;
-    int ix[3];
-    const int p = out_to_point<D>(s, t, ix);
+    int n, t, p;
+    if (!interp_item<D>(s, N, nP, n, t, p)) return;
     float y[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) y[a] = s.pts[((size_t)n * D + a) * nP + p];
@@ -918,11 +945,8 @@ __global__ __launch_bounds__(BLOCK) void k_interp_fwd(int N, int nP, Src s, Img 
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_interp_bwd(int N, int nP, Src s, Img im, const float* __restrict__ gout,
                                                       float* __restrict__ ggrid) {
-    const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (gid >= (size_t)N * nP) return;
-    const int n = gid / nP, t = gid % nP;
-    int ix[3];
-    const int p = out_to_point<D>(s, t, ix);
+    int n, t, p;
+    if (!interp_item<D>(s, N, nP, n, t, p)) return;
     float y[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) y[a] = s.pts[((size_t)n * D + a) * nP + p];
@@ -1793,7 +1817,8 @@ int cpab_interpolate_fwd(int ndim, const float* data, int N, int C, const int* d
     if (!grid || !out) return CPAB_ERR_ARG;
     s.pts = grid;
     const size_t total = (size_t)N * nP;
-    dim3 g((unsigned)((total + BLOCK - 1) / BLOCK));
+    const size_t blocks = ndim == 2 ? (size_t)N * ((s.g0 + 15) / 16) * ((s.g1 + 15) / 16) : (total + BLOCK - 1) / BLOCK;
+    dim3 g((unsigned)blocks);
 #define LAUNCH_IF(DD) hipLaunchKernelGGL((k_interp_fwd<DD>), g, dim3(BLOCK), 0, stream, N, nP, s, im, out);
     CPAB_DISPATCH_D(ndim, LAUNCH_IF)
 #undef LAUNCH_IF
@@ -1815,7 +1840,8 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
     im.gdata = grad_data;
     s.pts = grid;
     const size_t total = (size_t)N * nP;
-    dim3 g((unsigned)((total + BLOCK - 1) / BLOCK));
+    const size_t blocks = ndim == 2 ? (size_t)N * ((s.g0 + 15) / 16) * ((s.g1 + 15) / 16) : (total + BLOCK - 1) / BLOCK;
+    dim3 g((unsigned)blocks);
 #define LAUNCH_IB(DD) hipLaunchKernelGGL((k_interp_bwd<DD>), g, dim3(BLOCK), 0, stream, N, nP, s, im, grad_out, grad_grid);
     CPAB_DISPATCH_D(ndim, LAUNCH_IB)
 #undef LAUNCH_IB
