@@ -105,7 +105,11 @@ def uniform_meshgrid(ndim, domain_min, domain_max, n_points, device="cpu"):
     device = _device(device)
     lin = [torch.linspace(domain_min[i], domain_max[i], n_points[i], device=device) for i in range(ndim)]
     mesh = torch.meshgrid(lin[::-1], indexing="ij")
-    return torch.cat([g.reshape(1, -1) for g in mesh[::-1]], dim=0)
+    grid = torch.cat([g.reshape(1, -1) for g in mesh[::-1]], dim=0)
+    if ndim == 2:
+        # traversal hint for the grid transform (2D tiles); it changes no result
+        grid._cpab_grid_shape = (int(n_points[0]), int(n_points[1]))
+    return grid
 
 
 def transformer(points, theta, params):
@@ -115,7 +119,11 @@ def transformer(points, theta, params):
     if method != "closed_form":
         from .numeric import transformer_numeric
         return transformer_numeric(points, theta, params)
-    return TransformGrid.apply(points, theta, params, 1.0)
+    shape = getattr(points, "_cpab_grid_shape", None)
+    out = TransformGrid.apply(points, theta, params, 1.0, None, shape)
+    if shape is not None:
+        out._cpab_grid_shape = shape  # warped meshgrid points keep the meshgrid's order
+    return out
 
 
 def interpolate(ndim, data, grid, outsize):

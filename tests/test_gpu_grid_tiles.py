@@ -32,3 +32,19 @@ def test_tiled_grid_transform_equals_run_order(cuda, shape, per_theta, store):
     assert torch.equal(o1, o2)
     assert torch.allclose(g1, g2, rtol=1e-5, atol=1e-6 * g1.abs().max().item())
     assert torch.allclose(p1, p2, rtol=1e-5, atol=1e-6 * p1.abs().max().item())
+
+
+def test_meshgrid_hint_reaches_transform_grid(cuda):
+    """uniform_meshgrid tags 2D grids with their shape; transform_grid on such
+    a grid (and on its warped output) takes the tiled order, with the same
+    outputs as an untagged copy of the points."""
+    T, b = make_cpab("2d_4x4")
+    theta = (torch.randn(2, T.get_theta_dim(), generator=torch.Generator().manual_seed(2)) * 0.4).to(cuda)
+    grid = T.uniform_meshgrid([100, 37])
+    assert getattr(grid, "_cpab_grid_shape", None) == (100, 37)
+    plain = grid.clone()
+    out_tiled = T.transform_grid(grid, theta)
+    out_plain = T.transform_grid(plain, theta)
+    assert getattr(out_tiled, "_cpab_grid_shape", None) == (100, 37)
+    assert torch.equal(out_tiled, out_plain)
+    assert torch.equal(T.transform_grid(out_tiled, theta), T.transform_grid(out_plain, theta))
