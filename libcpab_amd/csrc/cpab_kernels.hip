@@ -60,17 +60,25 @@ struct Launch {
 
 // Tiling of a meshgrid output.  2D grids with a second axis of at least 8 go
 // in compact tiles (their warped footprint in the data is compact too, which
-// the backward accumulates in LDS); everything else in consecutive runs.
-__host__ __device__ inline int tile_ty(int ndim, int g1) {
-    if (ndim != 2) return 0;
-    return g1 >= 32 ? 32 : (g1 >= 16 ? 16 : (g1 >= 8 ? 8 : 0));
+// the backward accumulates in LDS); 3D grids whose last two axes are at least
+// 8 go in tiles of 8 x 8 lanes per wave (last two axes) stacked along the
+// first axis; everything else in consecutive runs.  The returned tile height
+// is 0 for runs.
+__host__ __device__ inline int tile_ty(int ndim, int g1, int g2 = 1) {
+    if (ndim == 2) return g1 >= 32 ? 32 : (g1 >= 16 ? 16 : (g1 >= 8 ? 8 : 0));
+    if (ndim == 3) return (g1 >= 8 && g2 >= 8) ? 8 : 0;
+    return 0;
 }
 
-__host__ __device__ inline int tile_count(int ndim, int nP, int g0, int g1, int ppt) {
-    const int ty = tile_ty(ndim, g1);
+__host__ __device__ inline int tile_count(int ndim, int nP, int g0, int g1, int ppt, int g2 = 1) {
+    const int ty = tile_ty(ndim, g1, g2);
     if (ty == 0) {
         const int n = (nP + BLOCK * ppt - 1) / (BLOCK * ppt);
         return n > 0 ? n : 1;
+    }
+    if (ndim == 3) {  // 8 x 8 lanes in (g1, g2), WAVES * ppt rows of g0
+        const int tx = ppt * WAVES;
+        return ((g0 + tx - 1) / tx) * ((g1 + 7) / 8) * ((g2 + 7) / 8);
     }
     const int tx = ppt * (BLOCK / ty);
     return ((g0 + tx - 1) / tx) * ((g1 + ty - 1) / ty);
@@ -102,10 +110,19 @@ __device__ __forceinline__ void block_exclusive_scan(int* v, int n) {
 }
 
 // output-order index t of round r of this thread in tile `tile`, -1 if outside
+template <int D>
 __device__ __forceinline__ int tile_t(const Launch& L, const Src& s, int tile, int r) {
     if (L.ty == 0) {
         const int t = (tile * L.ppt + r) * BLOCK + (int)threadIdx.x;
         return t < L.nP ? t : -1;
+    }
+    if constexpr (D == 3) {
+        const int tiles_z = (s.g2 + 7) >> 3, tiles_y = (s.g1 + 7) >> 3;
+        const int tz = tile % tiles_z, ty = (tile / tiles_z) % tiles_y, tx = tile / (tiles_z * tiles_y);
+        const int iz = tz * 8 + ((int)threadIdx.x & 7);
+        const int iy = ty * 8 + (((int)threadIdx.x >> 3) & 7);
+        const int ix = (tx * L.ppt + r) * WAVES + ((int)threadIdx.x >> 6);
+        return (ix < s.g0 && iy < s.g1 && iz < s.g2) ? (ix * s.g1 + iy) * s.g2 + iz : -1;
     }
     const int rows = BLOCK / L.ty;
     const int tiles_y = (s.g1 + L.ty - 1) / L.ty;
@@ -426,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
     const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
     for (int r = 0; r < L.ppt; ++r) {
-        const int p = tile_t(L, s, tile, r);  // point index (2D meshgrid tiles when L.ty > 0)
+        const int p = tile_t<D>(L, s, tile, r);  // point index (2D meshgrid tiles when L.ty > 0)
         if (p < 0) continue;
         float x[D], q[D];
 #pragma unroll
@@ -531,7 +548,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
     for (int r = 0; r < L.ppt; ++r) {
-        const int t = tile_t(L, s, tile, r);
+        const int t = tile_t<D>(L, s, tile, r);
         if (t < 0) continue;
         int ix[3];
         out_to_point<D>(s, t, ix);
@@ -613,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
     const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const float* pts = s.pts + (s.per_theta ? (size_t)th * D * L.nP : 0);
     for (int r = 0; r < L.ppt; ++r) {
-        const int p = tile_t(L, s, tile, r);  // point index (2D meshgrid tiles when L.ty > 0)
+        const int p = tile_t<D>(L, s, tile, r);  // point index (2D meshgrid tiles when L.ty > 0)
         const bool valid = p >= 0;
         float q0[D], mu[D];
         int c0 = 0;
@@ -716,7 +733,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             __syncthreads();
             int lo0 = INT_MAX, hi0 = INT_MIN, lo1 = INT_MAX, hi1 = INT_MIN;
             for (int r = 0; r < L.ppt; ++r) {
-                const int t = tile_t(L, s, tile, r);
+                const int t = tile_t<D>(L, s, tile, r);
                 if (t < 0) continue;
                 const float y[2] = {sv[t], sv[(size_t)L.nP + t]};
                 int i0[3], i1[3];
@@ -756,7 +773,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
         }
     }
     for (int r = 0; r < L.ppt; ++r) {
-        const int t = tile_t(L, s, tile, r);
+        const int t = tile_t<D>(L, s, tile, r);
         const bool valid = t >= 0;
         if (use_box && !valid) code[r * BLOCK + threadIdx.x] = -1;
         float q0[D], mu[D];
@@ -875,7 +892,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
 // saved (nullable) from the forward.  im.gdata (nullable, zero-filled by the
 // caller) receives dL/ddata by atomic scatter.
 template <int D, bool HAVES>
-__global__ __launch_bounds__(BLOCK) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
+__global__ __launch_bounds__(BLOCK, (D == 3 && HAVES) ? 4 : 1) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
                                                               const float* __restrict__ gout,
                                                               const float* __restrict__ saved, float* __restrict__ part,
                                                               int box_cap) {
@@ -1150,7 +1167,7 @@ __device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, flo
     L.nP = I->nP;
     L.tiles = I->tiles;
     L.ppt = ppt;
-    L.ty = tile_ty(nd, nd > 1 ? I->outsize[1] : 1);
+    L.ty = tile_ty(nd, nd > 1 ? I->outsize[1] : 1, nd > 2 ? I->outsize[2] : 1);
     L.t_end = t_end;
     g.ndim = nd;
     g.nc0 = I->tess.nc[0];
@@ -1536,7 +1553,7 @@ int pick_ppt(int n_theta, int nP) {
 // usual grids) whose data footprint fits the LDS box
 int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
     const int ppt = pick_ppt(n_theta, nP);
-    return (tile_ty(ndim, g1) > 0 && ppt > 2) ? 2 : ppt;
+    return (ndim == 2 && tile_ty(2, g1) > 0 && ppt > 2) ? 2 : ppt;
 }
 
 // LDS words for the dL/ddata bins and point slots of the fused 2D backward
@@ -1858,7 +1875,7 @@ size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* o
     int nP;
     if (src_out(t->ndim, outsize, s, nP)) return 0;
     const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
-    const size_t tiles = tile_count(t->ndim, nP, s.g0, s.g1, pick_ppt_bwd(t->ndim, n_theta, nP, s.g1));
+    const size_t tiles = tile_count(t->ndim, nP, s.g0, s.g1, pick_ppt_bwd(t->ndim, n_theta, nP, s.g1), s.g2);
     return align256((size_t)n_theta * Dt * 4) + align256((size_t)n_theta * tiles * Dt * 4);
 }
 
@@ -1879,13 +1896,13 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.n_theta = n_theta;
     L.nP = nP;
     L.ppt = pick_ppt(n_theta, nP);
-    L.ty = tile_ty(t->ndim, s.g1);
+    L.ty = tile_ty(t->ndim, s.g1, s.g2);
     const size_t lds = lds_tables(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     s.lin = lin;
-    L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt);
+    L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt, s.g2);
     L.t_end = t_end;
     Geo g = make_geo(t);
     dim3 grid(L.tiles * n_theta);
@@ -1917,7 +1934,7 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.n_theta = n_theta;
     L.nP = nP;
     L.ppt = pick_ppt_bwd(t->ndim, n_theta, nP, s.g1);
-    L.ty = tile_ty(t->ndim, s.g1);
+    L.ty = tile_ty(t->ndim, s.g1, s.g2);
     // 2D with saved state and dL/ddata: accumulate dL/ddata per tile in LDS
     const int box_cap = (t->ndim == 2 && L.ty > 0 && saved && grad_data &&
                          lds_bwd(t->ndim, t->nC) + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT)
@@ -1931,7 +1948,7 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     im.gdata = grad_data;
     s.lin = lin;
-    L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt);
+    L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt, s.g2);
     L.t_end = t_end;
     Geo g = make_geo(t);
     dim3 grid(L.tiles * n_theta);
@@ -2033,7 +2050,7 @@ size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, const int* outsize, int 
     int nP;
     if (src_out(t->ndim, outsize, s, nP)) return 0;
     const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
-    const size_t tiles = tile_count(t->ndim, nP, s.g0, s.g1, ppt);
+    const size_t tiles = tile_count(t->ndim, nP, s.g0, s.g1, ppt, s.g2);
     const size_t batched = align256(Dt * 4) + align256(tiles * Dt * 4);
     const size_t plain = cpab_data_workspace_bytes(t, 1, outsize);
     return batched > plain ? batched : plain;
@@ -2047,7 +2064,7 @@ int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, 
         int rc = check_tess(&I.tess);
         if (rc) return -rc;
         const int nd = I.tess.ndim;
-        item += tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt);
+        item += tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt, nd > 2 ? I.outsize[2] : 1);
         if (item > INT_MAX) return -CPAB_ERR_UNSUPPORTED;
     }
     if (!item_inst) return (int)item;  // count only
@@ -2056,7 +2073,7 @@ int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, 
     for (int i = 0; i < n_inst; ++i) {
         cpab_instance_t& I = inst[i];
         const int nd = I.tess.ndim;
-        const int tiles = tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt);
+        const int tiles = tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt, nd > 2 ? I.outsize[2] : 1);
         I.item0 = (int)item;
         I.tiles = tiles;
         for (int k = 0; k < tiles; ++k) item_inst[item + k] = i;

@@ -117,7 +117,8 @@ def test_interpolate_matches_restatement(cuda, ndim, size, outsize):
 
 
 @pytest.mark.parametrize("name,n_theta,scale,size", [("1d_10", 3, 0.8, [64]), ("2d_3x3", 3, 0.5, [24, 20]),
-                                                      ("3d_2x2x2", 2, 0.3, [9, 8, 7])])
+                                                      ("3d_2x2x2", 2, 0.3, [9, 8, 7]),
+                                                      ("3d_2x2x2", 2, 0.3, [12, 9, 10])])  # 3D tiles
 @pytest.mark.parametrize("store", [True, False])
 def test_transform_data_fused_matches_unfused(cuda, name, n_theta, scale, size, store):
     """Fused meshgrid->integrate->interpolate equals transform_grid + interpolate,
