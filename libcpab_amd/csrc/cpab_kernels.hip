@@ -65,7 +65,7 @@ struct Launch {
 // first axis; everything else in consecutive runs.  The returned tile height
 // is 0 for runs.
 __host__ __device__ inline int tile_ty(int ndim, int g1, int g2 = 1) {
-    if (ndim == 2) return g1 >= 32 ? 32 : (g1 >= 16 ? 16 : (g1 >= 8 ? 8 : 0));
+    if (ndim == 2) return g1 >= 16 ? 16 : (g1 >= 8 ? 8 : 0);  // 16: a wave is 4 x 16 pixels (32 and 8 measured slower)
     if (ndim == 3) return (g1 >= 8 && g2 >= 8) ? 8 : 0;
     return 0;
 }
@@ -1549,7 +1549,7 @@ int pick_ppt(int n_theta, int nP) {
     return 1;
 }
 
-// the fused backward of 2D tiled grids runs 512-point tiles (16 x 32 for the
+// the fused backward of 2D tiled grids runs 512-point tiles (32 x 16 for the
 // usual grids) whose data footprint fits the LDS box
 int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
     const int ppt = pick_ppt(n_theta, nP);
@@ -1558,7 +1558,7 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
 
 // LDS words for the dL/ddata bins and point slots of the fused 2D backward
 // (a 512-point tile needs 4096 + footprint cells + 1 words: up to ~2040
-// cells, a 16 x 32 tile at up to ~2x magnification)
+// cells, a 32 x 16 tile at up to ~2x magnification)
 constexpr int BOX_FLOATS = 6144;
 
 size_t lds_tables(int ndim, int nC) {

@@ -109,13 +109,14 @@ def test_workspace_and_saved_sizes():
     assert L.cpab_saved_floats(1, 8, 1024) == 8 * 2 * 1024
     assert L.cpab_jac_floats(2, 256, 65536) == 256 * (4 + 3) * 65536
     assert L.cpab_jac_floats(1, 8, 1024) == 8 * 1024
-    # transform_data: A_q plus one dL/dA partial per 2D tile (16 x 32 points in the backward)
+    # transform_data: A_q plus one dL/dA partial per 2D tile (32 x 16 points in the backward)
     osz = (ctypes.c_int * 3)(256, 256, 1)
     wd = L.cpab_data_workspace_bytes(ctypes.byref(t), 256, osz)
-    assert wd == 256 * D * 4 + 256 * (256 // 16) * (256 // 32) * D * 4
+    assert wd == 256 * D * 4 + 256 * (256 // 32) * (256 // 16) * D * 4
     # ragged grids still get whole tiles
     osz = (ctypes.c_int * 3)(100, 37, 1)
-    assert L.cpab_batch_workspace_bytes(ctypes.byref(t), osz, 1) >= 256 + (100 + 7) // 8 * 2 * D * 4
+    # (ppt 1: 16 x 16 tiles, ceil(100 / 16) x ceil(37 / 16) of them)
+    assert L.cpab_batch_workspace_bytes(ctypes.byref(t), osz, 1) >= 256 + ((100 + 15) // 16) * ((37 + 15) // 16) * D * 4
 
 
 def test_cpab_facade_contract():
