@@ -458,15 +458,38 @@ BWD_KERNEL = "k_transform_data_bwd<2, true>"
 TRAFFIC_FILE = "profiles/images2d_k_transform_data.traffic.json"
 
 
+VALU_FILE = "profiles/images2d_k_transform_data.valu.json"
+# wave64 VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz, 4 cycles per instruction
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def _profiled(path, kernel):
+    try:
+        with open(os.path.join(ROOT, path)) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
     1 GiB copy calibration) from the committed rocprofv3 counter summary of
     this workload, or None."""
-    try:
-        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
-            return json.load(f).get(kernel)
-    except (OSError, ValueError):
+    return _profiled(TRAFFIC_FILE, kernel)
+
+
+def valu_roofline(kernel, launch_ms):
+    """Issue-rate roofline of `kernel`, the bound the fused kernels actually
+    meet: its VALU wave-instructions per launch (SQ_INSTS_VALU from the
+    committed counter summary of this workload) over the launch time measured
+    in this run, against the nominal issue peak."""
+    n = _profiled(VALU_FILE, kernel)
+    if n is None:
         return None
+    rate = n / (launch_ms * 1e-3)
+    return {"kernel": kernel.split("<")[0], "bound": "valu-issue", "achieved": rate / 1e9,
+            "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s", "frac": rate / VALU_ISSUE_PEAK,
+            "instr_per_launch": n, "instr_source": VALU_FILE, "launch_ms": launch_ms}
 
 
 def main():
@@ -555,6 +578,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(BWD_KERNEL),
                      "traffic_source": TRAFFIC_FILE,
                      "algorithmic_bytes_per_launch": algo_bytes, "launch_ms": bwd_ms},
+        "valu": valu_roofline(BWD_KERNEL, bwd_ms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
