@@ -859,27 +859,38 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             if (cd >= 0) list[bins[cd & 2047] + ((cd >> 11) & 0x3FFFF)] = sl;
         }
         __syncthreads();
+        // Cell (i, j) is a corner of the points whose lower corner is in bins
+        // (i - a, j - b), a, b in {0, 1}.  Bins (i - a, j - 1) and (i - a, j)
+        // are adjacent in the list, so the four bins are two ranges, walked
+        // as one loop (its trip count is the wave's largest total, not the
+        // sum of four per-bin maxima): row a = 0 is [s0, e0) with bin
+        // (i, j - 1) below m0, row a = 1 is [s1, e1) split at m1.
         for (int k = threadIdx.x; k < cells; k += BLOCK) {
             const int i = k / bh, j = k - i * bh;
+            const int jb = j > 0 ? 1 : 0;
+            const int s0 = bins[k - jb], m0 = bins[k], e0 = bins[k + 1];
+            int s1 = 0, m1 = 0, e1 = 0;
+            if (i > 0) {
+                s1 = bins[k - bh - jb];
+                m1 = bins[k - bh];
+                e1 = bins[k - bh + 1];
+            }
+            const int n0 = e0 - s0, n = n0 + (e1 - s1);
             float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    if (i < a || j < b) continue;
-                    const int kb = k - a * bh - b;
-                    const int e1 = bins[kb + 1];
-                    for (int e = bins[kb]; e < e1; ++e) {
-                        const int sl = list[e];
-                        const float2 f = fd2[sl];
-                        const float4 gq = gv4[sl];
-                        const float w = (a ? f.x : 1.f - f.x) * (b ? f.y : 1.f - f.y);
-                        acc[0] = fmaf(w, gq.x, acc[0]);
-                        acc[1] = fmaf(w, gq.y, acc[1]);
-                        acc[2] = fmaf(w, gq.z, acc[2]);
-                        acc[3] = fmaf(w, gq.w, acc[3]);
-                    }
-                }
+            for (int u = 0; u < n; ++u) {
+                const bool row0 = u < n0;
+                const int e = row0 ? s0 + u : s1 + (u - n0);
+                const int sl = list[e];
+                const float2 f = fd2[sl];
+                const float4 gq = gv4[sl];
+                const float wx = row0 ? 1.f - f.x : f.x;
+                const float wy = e < (row0 ? m0 : m1) ? f.y : 1.f - f.y;
+                const float w = wx * wy;
+                acc[0] = fmaf(w, gq.x, acc[0]);
+                acc[1] = fmaf(w, gq.y, acc[1]);
+                acc[2] = fmaf(w, gq.z, acc[2]);
+                acc[3] = fmaf(w, gq.w, acc[3]);
+            }
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 if (c < im.C && acc[c] != 0.f)
