@@ -17,6 +17,8 @@ saltation term) fails it:
   * interpolation against torch.nn.functional.grid_sample and finite
     differences.
 """
+import itertools
+
 import numpy as np
 import pytest
 import torch
@@ -373,6 +375,27 @@ def test_interpolation_matches_grid_sample_2d():
     assert np.abs(mine - s.numpy()).max() < 1e-12
 
 
+@pytest.mark.parametrize("nd,size,outsize", [(1, [40], [57]), (2, [13, 17], [11, 19]), (3, [7, 6, 5], [5, 8, 6])])
+def test_torch_interpolation_reference_equals_the_oracle(nd, size, outsize):
+    """The plain-torch restatement the fused GPU path is tested against gives
+    the oracle's values and both gradients (data, grid), in FP64, also for
+    grids outside [0,1] (both corners clamp)."""
+    from torch_interp_ref import torch_interpolate
+    rng = np.random.default_rng(23 + nd)
+    data = rng.random((2, 3, *size))
+    grid = rng.uniform(-0.2, 1.2, (2, nd, int(np.prod(outsize))))
+    ref = O.interpolate(nd, data, grid, outsize)
+    go = rng.standard_normal(ref.shape)
+    gd_ref, gg_ref = O.interpolate_backward(nd, data, grid, outsize, go)
+    dt = torch.tensor(data, requires_grad=True)
+    gt = torch.tensor(grid, requires_grad=True)
+    out = torch_interpolate(dt, gt, outsize)
+    (out * torch.tensor(go)).sum().backward()
+    assert np.abs(out.detach().numpy() - ref).max() < 1e-12
+    assert np.abs(dt.grad.numpy() - gd_ref).max() < 1e-12
+    assert np.abs(gt.grad.numpy() - gg_ref).max() < 1e-12
+
+
 def test_interpolation_identity_and_linear():
     rng = np.random.default_rng(21)
     data = rng.uniform(size=(1, 2, 6, 5, 4))
@@ -407,3 +430,27 @@ def test_interpolation_backward_matches_finite_differences(ndim):
     e = rng.standard_normal(data.shape)
     lhs = (O.interpolate(ndim, e, grid, outsize) * go).sum()
     assert abs(lhs - (gd * e).sum()) < 1e-10 * max(1, abs(lhs))
+
+
+def test_touching_mass_bounds_a_weight_perturbation():
+    """touching_mass (the per-cell bound the GPU dL/ddata check uses) is the
+    sum of |grad_out| over the points reading each cell: it equals the data
+    gradient of <|grad_out|, interpolate> with every corner weight set to 1,
+    checked here against a direct loop."""
+    from torch_interp_ref import touching_mass
+    rng = np.random.default_rng(29)
+    for nd, size, outsize in [(1, [9], [7]), (2, [5, 4], [3, 6]), (3, [3, 4, 2], [2, 3, 4])]:
+        nP = int(np.prod(outsize))
+        grid = rng.uniform(-0.2, 1.2, (2, nd, nP))
+        g = rng.standard_normal((2, 3, *outsize))
+        got = touching_mass(torch.tensor(grid), size, outsize, torch.tensor(np.abs(g))).numpy()
+        want = np.zeros((2, 3, *size))
+        gp = np.abs(g).transpose(0, 1, *range(nd + 1, 1, -1)).reshape(2, 3, nP)  # x fastest
+        for n in range(2):
+            for p in range(nP):
+                lo = [min(max(int(np.floor(grid[n, a, p] * (size[a] - 1))), 0), size[a] - 1) for a in range(nd)]
+                hi = [min(max(int(np.floor(grid[n, a, p] * (size[a] - 1))) + 1, 0), size[a] - 1) for a in range(nd)]
+                for bits in itertools.product((0, 1), repeat=nd):
+                    idx = tuple(hi[a] if bits[a] else lo[a] for a in range(nd))
+                    want[(n, slice(None)) + idx] += gp[n, :, p]
+        assert np.allclose(got, want, rtol=0, atol=1e-12)
