@@ -150,3 +150,16 @@ def test_meshgrid_order_matches_reference():
     g = uniform_meshgrid(2, [0, 0], [1, 1], [4, 3]).numpy()
     assert np.allclose(g, O.uniform_meshgrid([4, 3]))
     assert np.allclose(g[:, :4], [[0, 1 / 3, 2 / 3, 1], [0, 0, 0, 0]])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 (or a multi-rank launch)
+    exits with status 2 and says so, before touching a GPU."""
+    import subprocess
+    import sys
+    for args, env in [(["--gpus", "2"], {}), (["--gpus", "1"], {"WORLD_SIZE": "4"})]:
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, capture_output=True, text=True,
+                           env=dict(os.environ, **env), timeout=120)
+        assert r.returncode == 2
+        assert "one GPU" in r.stderr
+        assert r.stdout == ""
