@@ -454,3 +454,22 @@ def test_touching_mass_bounds_a_weight_perturbation():
                     idx = tuple(hi[a] if bits[a] else lo[a] for a in range(nd))
                     want[(n, slice(None)) + idx] += gp[n, :, p]
         assert np.allclose(got, want, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,zb", [([3, 3], True), ([2, 2], False), ([2, 2, 2], True)])
+def test_volume_preserving_basis_is_trace_free(nc, zb):
+    """volume_perservation=True adds one zero-trace row per cell
+    (libcpab/core/tesselation.py:155-162): every cell's velocity matrix of
+    every theta in span(B) is trace-free, B stays an orthonormal null-space
+    basis, and the space is strictly smaller than without the constraint."""
+    t = make_tessellation(nc, zero_boundary=zb, volume_perservation=True)
+    t0 = make_tessellation(nc, zero_boundary=zb)
+    ndim = len(nc)
+    assert 0 < t.B.shape[1] < t0.B.shape[1]
+    assert np.abs(t.L @ t.B).max() < 1e-9
+    assert np.allclose(t.B.T @ t.B, np.eye(t.B.shape[1]), atol=1e-10)
+    theta = RNG.standard_normal((5, t.B.shape[1]))
+    A = (t.B @ theta.T).T.reshape(5, -1, ndim, ndim + 1)
+    assert np.abs(np.trace(A[..., :ndim], axis1=2, axis2=3)).max() < 1e-12
+    A0 = (t0.B @ RNG.standard_normal((5, t0.B.shape[1])).T).T.reshape(5, -1, ndim, ndim + 1)
+    assert np.abs(np.trace(A0[..., :ndim], axis1=2, axis2=3)).max() > 1e-3
