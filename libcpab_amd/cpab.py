@@ -171,9 +171,19 @@ class Cpab(object):
         return self.backend.to(samples, device=self.device)
 
     def sample_transformation_with_prior(self, n_sample=1, mean=None, length_scale=0.1, output_variance=1):
-        """Smooth prior over cell parameters (libcpab/cpab.py:192-241): squared
-        exponential kernel over cell-centre distances on the diagonal of each
-        cell block, a large distance elsewhere, pushed to theta space by B."""
+        """Smooth prior over cell parameters (libcpab/cpab.py:192-241): samples
+        of N(mean, prior_covariance(length_scale, output_variance))."""
+        import torch
+        cov_theta = self.prior_covariance(length_scale, output_variance).to(torch.float32)
+        dev = self.backend._device(self.device)
+        return self.sample_transformation(n_sample, mean=mean, cov=cov_theta.to(dev))
+
+    def prior_covariance(self, length_scale=0.1, output_variance=1):
+        """The prior's covariance in theta space [d, d] (float64, host): squared
+        exponential kernel over squared cell-centre distances on the diagonal
+        of each cell block of the [D, D] parameter covariance, 100 x the
+        largest distance elsewhere, pushed to theta space as B^T C B
+        (libcpab/cpab.py:211-234, built here without the nC^2 block loop)."""
         import torch
         p = self.params
         centers = torch.tensor(self._centers, dtype=torch.float64)
@@ -188,9 +198,7 @@ class Cpab(object):
         cov_init = blocks.permute(0, 2, 1, 3).reshape(p.D, p.D)
         cov_avees = output_variance ** 2 * torch.exp(-(cov_init / (2 * length_scale ** 2)))
         B = torch.tensor(p.basis, dtype=torch.float64)
-        cov_theta = (B.t() @ cov_avees @ B).to(torch.float32)
-        dev = self.backend._device(self.device)
-        return self.sample_transformation(n_sample, mean=mean, cov=cov_theta.to(dev))
+        return B.t() @ cov_avees @ B
 
     def identity(self, n_sample=1, epsilon=0):
         return self.backend.identity(self.params.d, n_sample, epsilon, self.device)

@@ -163,3 +163,31 @@ def test_bench_refuses_more_than_one_gpu():
         assert r.returncode == 2
         assert "one GPU" in r.stderr
         assert r.stdout == ""
+
+
+@pytest.mark.parametrize("nc,ls,ov", [([3, 3], 0.1, 1.0), ([4], 0.3, 2.0), ([2, 2, 2], 0.2, 0.5)])
+def test_prior_covariance_equals_the_block_loop(nc, ls, ov):
+    """prior_covariance (vectorised) equals the reference's nC x nC block
+    loop (libcpab/cpab.py:211-234) written out here: squared cell-centre
+    distances on each block's diagonal, 100 x the largest elsewhere, squared
+    exponential kernel, B^T C B; symmetric and positive semi-definite."""
+    from libcpab_amd import Cpab
+    T = Cpab(nc, backend="pytorch", device="gpu")
+    p = T.params
+    c = np.asarray(T._centers, dtype=np.float64)
+    dist = ((c[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+    ppc = p.params_pr_cell
+    C = np.zeros((p.D, p.D))
+    for i in range(p.nC):
+        for j in range(p.nC):
+            block = np.full((ppc, ppc), 100 * dist.max())
+            block[np.arange(ppc), np.arange(ppc)] = dist[i, j]
+            C[ppc * i:ppc * (i + 1), ppc * j:ppc * (j + 1)] = block
+    K = ov ** 2 * np.exp(-C / (2 * ls ** 2))
+    want = p.basis.T @ K @ p.basis
+    got = T.prior_covariance(ls, ov).numpy()
+    assert got.shape == (p.d, p.d)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    assert np.allclose(got, got.T, atol=1e-12 * np.abs(got).max())
+    assert np.linalg.eigvalsh(got).min() > -1e-9 * np.abs(got).max()
+    assert np.allclose(T.prior_covariance(ls, 2 * ov).numpy(), 4 * got, rtol=1e-12)
