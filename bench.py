@@ -140,6 +140,7 @@ def cpu_baseline(T, theta, data, target, size, npts, nerr, seed=1):
     gA, _ = O.cf_backward(2, p.nc, A64, p.facet_planes, p.facet_nbr, pts, w.cpu().double().numpy())
     gth = gA.reshape(1, -1) @ p.basis
     res["max_rel_err_grad"] = float(np.abs(th.grad.cpu().double().numpy() - gth).max() / np.abs(gth).max())
+    res["max_rel_err_grad_fd"] = fd_gradient_error(T, theta, pts)
     # --- reference CPU op on npts pixels ---
     if O.ref_lib() is None:
         return res, None
@@ -204,6 +205,31 @@ def oracle_error(T, theta, pts, seed=2):
     gth = gA.reshape(1, -1) @ p.basis
     return (float(np.abs(gpu.detach().cpu().double().numpy() - exact).max()),
             float(np.abs(th.grad.cpu().double().numpy() - gth).max() / np.abs(gth).max()))
+
+
+def fd_gradient_error(T, theta, pts, n=1024, eps=1e-5, seed=3):
+    """Max |dL/dtheta (GPU) - FP64 central differences| / max |FD| for
+    L = <w, T_theta(pts[:, :n])>, random w, theta[0]: the differences are of
+    the oracle's exact FP64 flow, one pair per theta direction."""
+    import numpy as np
+    import torch
+    import oracle as O
+    p = T.get_params()
+    nd = p.ndim
+    pts = pts[:, :n]
+    w = np.random.default_rng(seed).standard_normal((1, nd, pts.shape[1]))
+    th = theta[0:1].detach().clone().requires_grad_(True)
+    gpu = T.transform_grid(torch.tensor(pts, dtype=torch.float32, device=theta.device), th)
+    (gpu * torch.tensor(w, dtype=torch.float32, device=theta.device)).sum().backward()
+    th0 = theta[0:1].detach().cpu().double().numpy()
+    fd = np.empty(th0.shape[1])
+    for k in range(th0.shape[1]):
+        e = np.zeros_like(th0)
+        e[0, k] = eps
+        fp = O.cf_forward(nd, p.nc, O.expand_A(p.basis, th0 + e, nd), p.facet_planes, p.facet_nbr, pts)
+        fm = O.cf_forward(nd, p.nc, O.expand_A(p.basis, th0 - e, nd), p.facet_planes, p.facet_nbr, pts)
+        fd[k] = ((fp - fm) * w).sum() / (2 * eps)
+    return float(np.abs(th.grad.cpu().double().numpy()[0] - fd).max() / np.abs(fd).max())
 
 
 def config_inputs(name, device, seed=0):
@@ -305,6 +331,7 @@ def run_config(args):
     rng = np.random.default_rng(1)
     pts = mesh[:, np.sort(rng.choice(mesh.shape[1], size=min(args.err_points, mesh.shape[1]), replace=False))]
     errs = [oracle_error(T, th, pts) for T, th in zip(Ts, thetas)]
+    fd_errs = [fd_gradient_error(T, th, pts) for T, th in zip(Ts, thetas)]
     line = {
         "metric": "fwd+bwd points/s (%s); max err vs FP64 oracle" % args.workload,
         "value": points / sec, "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -316,7 +343,9 @@ def run_config(args):
         "batched_over_plain": (points / sec / plain["value"]) if plain else None,
         "max_err_fwd": max(e[0] for e in errs),
         "max_rel_err_grad": max(e[1] for e in errs),
-        "err_sample": "theta[0] of each stage, %d grid points" % pts.shape[1],
+        "max_rel_err_grad_fd": max(fd_errs),
+        "err_sample": "theta[0] of each stage, %d grid points (finite differences: the first %d)" % (
+            pts.shape[1], min(1024, pts.shape[1])),
     }
     print(json.dumps(line), flush=True)
 
@@ -589,6 +618,7 @@ def main():
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "max_err_fwd": res.get("max_err_fwd"),
         "max_rel_err_grad": res.get("max_rel_err_grad"),
+        "max_rel_err_grad_fd": res.get("max_rel_err_grad_fd"),
         "roofline": {"kernel": "k_transform_data_bwd", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(BWD_KERNEL),
                      "traffic_source": TRAFFIC_FILE,
