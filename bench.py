@@ -40,27 +40,42 @@ def make_inputs(T, n_theta, C, size, device, seed=0):
     return theta.to(device), data.to(device), target.to(device)
 
 
-def run_steps(T, theta, data, target, size, steps):
+def timed_loop(fn, steps, warmup):
+    """Mean seconds per call of fn over `steps` calls bracketed by device
+    synchronisation, and the median of the per-call durations from HIP events
+    recorded between the calls on the current stream (no synchronisation
+    inside the timed region)."""
+    import statistics
     import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i].record()
+        fn()
+    ev[steps].record()
+    torch.cuda.synchronize()
+    mean = (time.perf_counter() - t0) / steps
+    med = statistics.median(ev[i].elapsed_time(ev[i + 1]) for i in range(steps)) * 1e-3
+    return mean, med
+
+
+def time_steps(T, theta, data, target, size, steps, warmup):
+    """(mean, median) seconds per fwd+bwd step of the image workload."""
     th = theta.detach().clone().requires_grad_(True)
     dd = data.detach().clone().requires_grad_(True)
-    for _ in range(steps):
+
+    def step():
+        import torch
         th.grad = None
         dd.grad = None
         out = T.transform_data(dd, th, size)
         loss = torch.nn.functional.mse_loss(out, target)
         loss.backward()
-    return th, dd
 
-
-def time_steps(T, theta, data, target, size, steps, warmup):
-    import torch
-    run_steps(T, theta, data, target, size, warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run_steps(T, theta, data, target, size, steps)
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps
+    return timed_loop(step, steps, warmup)
 
 
 def time_bwd_kernel(T, theta, data, target, size, reps):
@@ -309,21 +324,14 @@ def run_config(args):
         return ths
 
     def timed(steps, warmup):
-        for _ in range(warmup):
-            step()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            step()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps
+        return timed_loop(step, steps, warmup)
 
-    sec = timed(args.steps, args.warmup)
+    sec, med = timed(args.steps, args.warmup)
     log("batched: %.3f ms/step, %.3e points/s" % (sec * 1e3, points / sec))
     plain = None
     if not args.no_plain and not seq:
         Ts[0].set_solver_params(launch="plain")
-        psec = timed(args.plain_steps, 1)
+        psec, _ = timed(args.plain_steps, 1)
         Ts[0].set_solver_params(launch="batched")
         plain = {"value": points / psec, "ms_per_step": psec * 1e3}
         log("plain: %.3f ms/step" % (psec * 1e3))
@@ -335,8 +343,8 @@ def run_config(args):
     line = {
         "metric": "fwd+bwd points/s (%s); max err vs FP64 oracle" % args.workload,
         "value": points / sec, "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "ms_per_step": sec * 1e3, "ms_per_step_median": med * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "n_theta": n_theta, "outsize": size, "points_per_step": points,
                    "stages": len(Ts)},
         "plain": plain,
@@ -539,7 +547,7 @@ def valu_roofline(kernel, launch_ms):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=5)
@@ -575,14 +583,14 @@ def main():
     points = n_theta * size[0] * size[1]
     log("workload ready: %d thetas x %d points" % (n_theta, size[0] * size[1]))
 
-    sec = time_steps(T, theta, data, target, size, args.steps, args.warmup)
+    sec, med = time_steps(T, theta, data, target, size, args.steps, args.warmup)
     value = points / sec
     log("batched: %.3f ms/step, %.3e points/s" % (sec * 1e3, value))
 
     plain = None
     if not args.no_plain:
         T.set_solver_params(launch="plain")
-        psec = time_steps(T, theta, data, target, size, args.plain_steps, 1)
+        psec, _ = time_steps(T, theta, data, target, size, args.plain_steps, 1)
         T.set_solver_params(launch="batched")
         plain = {"value": points / psec, "ms_per_step": psec * 1e3}
         log("plain: %.3f ms/step, %.3e points/s" % (psec * 1e3, points / psec))
@@ -605,6 +613,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": sec * 1e3,
+        "ms_per_step_median": med * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
