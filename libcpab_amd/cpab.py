@@ -58,14 +58,17 @@ class Cpab(object):
 
     Arguments:
         tess_size: list, number of cells per dimension (1, 2 or 3 entries)
-        backend: 'pytorch' (this build; 'numpy' / 'tensorflow' are not provided)
-        device: 'gpu' ('cpu' is not provided by this build)
+        backend: 'numpy' (the reference's default), 'tensorflow' or 'pytorch';
+            this build provides 'pytorch' only
+        device: 'cpu' (the reference's default) or 'gpu'; this build provides
+            'gpu' only, so Cpab(tess_size) with the defaults raises
+            NotImplementedError as the other unprovided combinations do
         zero_boundary: bool, zero normal velocity on the domain boundary
         volume_perservation: bool, trace-free cell fields
         override: bool, rebuild the cached basis
     """
 
-    def __init__(self, tess_size, backend="pytorch", device="gpu", zero_boundary=True,
+    def __init__(self, tess_size, backend="numpy", device="cpu", zero_boundary=True,
                  volume_perservation=False, override=False, basis=None):
         self._check_input(tess_size, backend, device, zero_boundary, volume_perservation, override)
         p = params()

@@ -23,7 +23,7 @@ PROBLEMS = [  # tess, zero_boundary, channels, data size, output size
 
 def _make(cuda, seed=0, save=True, problems=PROBLEMS):
     g = torch.Generator().manual_seed(seed)
-    cpabs = [Cpab(nc, zero_boundary=zb) for nc, zb, _, _, _ in problems]
+    cpabs = [Cpab(nc, backend="pytorch", device="gpu", zero_boundary=zb) for nc, zb, _, _, _ in problems]
     probs = [(T, c, s, o) for T, (_, _, c, s, o) in zip(cpabs, problems)]
     b = CpabBatch(probs, device=cuda, save=save)
     for i, (T, c, s, o) in enumerate(probs):

@@ -131,6 +131,8 @@ def test_cpab_facade_contract():
         Cpab([0, 3])
     with pytest.raises(NotImplementedError):
         Cpab([3, 3], backend="numpy", device="cpu")
+    with pytest.raises(NotImplementedError):  # the reference's defaults: numpy on the CPU
+        Cpab([3, 3])
     with pytest.raises(NotImplementedError):
         Cpab([3, 3], backend="pytorch", device="cpu")
     # tensors on the wrong device are rejected like in the reference
@@ -140,7 +142,7 @@ def test_cpab_facade_contract():
     assert T.params.nstepsolver == 20 and T.params.method == "numeric"
     with pytest.raises(AssertionError):
         T.set_solver_params(nstepsolver=0)
-    S = CpabSequential(T, Cpab([4, 4]))
+    S = CpabSequential(T, Cpab([4, 4], backend="pytorch", device="gpu"))
     assert S.get_theta_dim() == [34, 62]
 
 
