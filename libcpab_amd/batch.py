@@ -89,6 +89,10 @@ class CpabBatch:
         self.grad_out_views = [self.grad_out[o:o + k].view([c] + s) for o, k, c, s in zip(o_out, n_out, C, osz)]
         self.grad_data_views = [self.grad_data[o:o + k].view([c] + s) for o, k, c, s in zip(o_da, n_data, C, dsz)]
         self.grad_theta_views = [self.grad_theta[o:o + k] for o, k in zip(o_th, d)]
+        # saved state of the forward per problem, [rows, nP] in output order
+        # (rows 0..ndim-1: the warped points in x-space); None without save
+        self.saved_views = [self.saved[o:o + k].view(-1, p) if save else None
+                            for o, k, p in zip(o_sv, n_sv, nP)]
         self.points = sum(nP)
         self.used = {"data": n_data, "out": n_out}
 

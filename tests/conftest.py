@@ -38,3 +38,39 @@ def make_cpab(name):
     b = load_basis(name)
     nc = [int(v) for v in b["nc"]]
     return Cpab(nc, backend="pytorch", device="gpu", zero_boundary=bool(b["zero_boundary"]), basis=b["B"]), b
+
+
+def _chunks(n, k):
+    step = max(1, (n + k - 1) // k)
+    return [(i, min(i + step, n)) for i in range(0, n, step)]
+
+
+def par_cf_forward(ndim, nc, A, planes, nbr, points, workers=8):
+    """oracle.cf_forward over point slices on a thread pool (the C oracle runs
+    without the GIL); points [ndim, nP] shared by every theta of A."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle as O
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    parts = _chunks(points.shape[-1], workers)
+    with ThreadPoolExecutor(len(parts)) as ex:
+        outs = list(ex.map(lambda r: O.cf_forward(ndim, nc, A, planes, nbr,
+                                                  np.ascontiguousarray(points[:, r[0]:r[1]])), parts))
+    return np.concatenate(outs, axis=-1)
+
+
+def par_cf_backward(ndim, nc, A, planes, nbr, points, grad_out, workers=8):
+    """oracle.cf_backward over point slices on a thread pool: dL/dA summed
+    over the slices, dL/dpoints concatenated."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle as O
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    grad_out = np.asarray(grad_out, dtype=np.float64)
+    parts = _chunks(points.shape[-1], workers)
+
+    def one(r):
+        return O.cf_backward(ndim, nc, A, planes, nbr, np.ascontiguousarray(points[:, r[0]:r[1]]),
+                             np.ascontiguousarray(grad_out[..., r[0]:r[1]]))
+
+    with ThreadPoolExecutor(len(parts)) as ex:
+        res = list(ex.map(one, parts))
+    return sum(r[0] for r in res), np.concatenate([r[1] for r in res], axis=-1)
