@@ -3,7 +3,10 @@
 Run in the build container (needs /root/reference for the reference's own
 CPU op, compiled by oracle/Makefile into oracle/_ref/):
 
-    make -C oracle && python tests/golden/make_fixtures.py
+    make -C oracle && python tests/golden/make_fixtures.py [--force]
+
+Existing files are kept unless --force is given (a rewrite changes the
+archives' timestamps even when the numbers are the same).
 
 Fixtures (all .npz, no pickles):
   basis_<name>.npz   L-derived basis B [D, d] (float64), facet planes and
@@ -36,6 +39,7 @@ TESS = {
     "2d_4x4": ([4, 4], True),
     "2d_2x2_nzb": ([2, 2], False),
     "3d_2x2x2": ([2, 2, 2], True),
+    "3d_3x3x3": ([3, 3, 3], True),
 }
 
 STEP_CASES = {  # name: (tess, n_theta, nP, theta scale)
@@ -48,13 +52,18 @@ STEP_CASES = {  # name: (tess, n_theta, nP, theta scale)
 def main():
     if O.ref_lib() is None:
         raise SystemExit("oracle/_ref/libcpab_ref.so missing: run `make -C oracle` with /root/reference present")
+    force = "--force" in sys.argv[1:]
     for name, (nc, zb) in TESS.items():
+        if os.path.exists(os.path.join(HERE, "basis_%s.npz" % name)) and not force:
+            continue
         t = make_tessellation(nc, zero_boundary=zb)
         planes, nbr = t.facet_table()
         np.savez_compressed(os.path.join(HERE, "basis_%s.npz" % name), nc=np.array(nc), zero_boundary=zb,
                             B=t.B, planes=planes, nbr=nbr)
     rng = np.random.default_rng(20240601)
     for name, (tname, n_theta, nP, scale) in STEP_CASES.items():
+        if os.path.exists(os.path.join(HERE, "ref_step_%s.npz" % name)) and not force:
+            continue
         nc, zb = TESS[tname]
         ndim = len(nc)
         f = np.load(os.path.join(HERE, "basis_%s.npz" % tname))
