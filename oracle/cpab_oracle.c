@@ -28,7 +28,8 @@
  *      matrix S = I + (v_next - v_prev) n^T / (n . v_prev) -- the hit-time and
  *      cell-change terms, identically I when the field is continuous.
  *
- * Parity status per function (what pins it) is listed in oracle/README.md.
+ * What pins each function (closed forms, invariants, the reference's compiled
+ * op, finite differences) is tested in tests/test_oracle.py.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -84,7 +84,7 @@ def test_lookup_restatement_matches_compiled_reference(nc):
 # geometry
 # --------------------------------------------------------------------------
 @pytest.mark.parametrize("nc,zb,d", [([10], True, 9), ([50], True, 49), ([3, 3], True, 34), ([4, 4], True, 62),
-                                     ([2, 2], False, 20), ([2, 2, 2], True, 27)])
+                                     ([2, 2], False, 20), ([2, 2, 2], True, 27), ([3, 3, 3], True, 96)])
 def test_basis_dimension_and_null_space(nc, zb, d):
     t = make_tessellation(nc, zero_boundary=zb)
     assert t.B.shape[1] == d
@@ -92,7 +92,7 @@ def test_basis_dimension_and_null_space(nc, zb, d):
     assert np.allclose(t.B.T @ t.B, np.eye(d), atol=1e-10)
 
 
-@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_4x4", "2d_2x2_nzb", "3d_2x2x2"])
+@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_4x4", "2d_2x2_nzb", "3d_2x2x2", "3d_3x3x3"])
 def test_fixture_basis_spans_the_rebuilt_null_space(name):
     b = load_basis(name)
     t = make_tessellation([int(v) for v in b["nc"]], zero_boundary=bool(b["zero_boundary"]))
@@ -473,3 +473,78 @@ def test_volume_preserving_basis_is_trace_free(nc, zb):
     assert np.abs(np.trace(A[..., :ndim], axis1=2, axis2=3)).max() < 1e-12
     A0 = (t0.B @ RNG.standard_normal((5, t0.B.shape[1])).T).T.reshape(5, -1, ndim, ndim + 1)
     assert np.abs(np.trace(A0[..., :ndim], axis1=2, axis2=3)).max() > 1e-3
+
+
+# --------------------------------------------------------------------------
+# the numpy-backend restatement timed as the "ref-numpy" CPU baseline
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nc", [[7], [50], [3, 3], [4, 2], [2, 2, 2], [3, 3, 3]])
+def test_numpy_path_lookup_equals_the_oracle_lookup(nc):
+    from oracle import numpy_path as NP
+    ndim = len(nc)
+    pts = RNG.uniform(-0.3, 1.3, size=(ndim, 20000))
+    # points on grid lines, vertices and diagonals
+    k = RNG.integers(0, nc, size=(2000, ndim)).T
+    t = RNG.uniform(0, 1, size=(1, 2000))
+    grid = O.uniform_meshgrid([2 * n + 1 for n in nc])
+    pts = np.concatenate([pts, (k + t) / np.array(nc)[:, None], grid], axis=1)
+    assert np.array_equal(NP.findcellidx(ndim, pts, nc), O.findcell(ndim, nc, pts))
+
+
+@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_2x2_nzb", "3d_2x2x2"])
+def test_numpy_path_transformer_equals_the_fp64_step_restatement(name):
+    from oracle import numpy_path as NP
+    b, nc, ndim, A, theta = _setup(name, 2, 0.5, seed=8)
+    lo, hi = (0.0, 1.0) if bool(b["zero_boundary"]) else (-0.1, 1.1)
+    pts = RNG.uniform(lo, hi, size=(ndim, 500))
+    got = NP.transformer(pts, theta, b["B"], nc, 50)
+    ref = O.step_forward_f64(ndim, nc, 50, pts, O.trels(A, 50))
+    assert np.abs(got - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("ndim,size,outsize", [(1, [40], [57]), (2, [13, 17], [11, 19]), (3, [7, 6, 5], [5, 8, 6])])
+def test_numpy_path_interpolation_equals_the_nc_restatement(ndim, size, outsize):
+    """NWHC layout of libcpab/numpy/interpolation.py against the NC layout of
+    the torch backend's interpolation, restated independently."""
+    from oracle import numpy_path as NP
+    data = RNG.uniform(size=(2, 3) + tuple(size))
+    grid = RNG.uniform(-0.1, 1.1, size=(2, ndim, int(np.prod(outsize))))
+    nwhc = np.moveaxis(data, 1, -1)
+    got = NP.interpolate(ndim, nwhc, grid, outsize)
+    ref = np.moveaxis(O.interpolate(ndim, data, grid, outsize), 1, -1)
+    assert np.abs(got - ref).max() < 1e-12
+
+
+# --------------------------------------------------------------------------
+# velocity field
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("nc", [[5], [3, 3], [2, 2, 2]])
+def test_vectorfield_of_a_global_affine_field_is_exact(nc):
+    """Every cell carrying the same affine map: v = M x + b at every point,
+    whatever cell the lookup picks (outside the domain too)."""
+    ndim = len(nc)
+    rng = np.random.default_rng(21)
+    M = rng.standard_normal((ndim, ndim))
+    bvec = rng.standard_normal(ndim)
+    nC = O.n_cells(ndim, nc)
+    A = np.tile(np.concatenate([M, bvec[:, None]], 1), (nC, 1, 1))
+    pts = rng.uniform(-0.3, 1.3, size=(ndim, 2000))
+    v = O.vectorfield(ndim, nc, A, pts)
+    assert np.abs(v - (M @ pts + bvec[:, None])).max() < 1e-13
+
+
+@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("2d_3x3", 0.5), ("2d_2x2_nzb", 0.5), ("3d_2x2x2", 0.3)])
+def test_vectorfield_is_the_time_derivative_of_the_flow(name, scale):
+    """(phi_eps(x) - x) / eps -> v(x): the field of the oracle's lookup is the
+    one its exact integrator follows (first-order difference, error O(eps))."""
+    b, nc, ndim, A, theta = _setup(name, 1, scale, seed=22)
+    lo, hi = (0.02, 0.98) if bool(b["zero_boundary"]) else (-0.1, 1.1)
+    pts = RNG.uniform(lo, hi, size=(ndim, 400))
+    v = O.vectorfield(ndim, nc, A[0], pts)
+    errs = []
+    for eps in (1e-4, 1e-5):
+        d = (O.cf_forward(ndim, nc, A, b["planes"], b["nbr"], pts, T=eps)[0] - pts) / eps
+        errs.append(np.abs(d - v).max())
+    scale_v = max(np.abs(v).max(), 1.0)
+    assert errs[1] < 1e-4 * scale_v
+    assert errs[1] < 0.2 * errs[0] + 1e-9  # shrinks with eps
