@@ -201,6 +201,58 @@ def cpu_baseline(T, theta, data, target, size, npts, nerr, seed=1):
                  "affinity_cpus": affinity, "torch_threads": torch.get_num_threads()}
 
 
+def numpy_baseline(T, theta, data, size, npts, seed=1, reps=5):
+    """The reference's numpy backend -- FP64 fixed-step integration
+    (nstepsolver = 50, scipy expm per cell) and NWHC linear interpolation,
+    libcpab/numpy/transformer.py:22-61 and libcpab/numpy/interpolation.py --
+    restated in vectorised numpy (oracle/numpy_path.py), timed on one core:
+    (a) the plain [3,3] workload whole (one theta ~ N(0, 0.5^2 I), seed 0,
+    100x100 meshgrid, one 100x100x3 image), median of `reps` runs; (b)
+    theta[0] x npts random pixels of the image workload.  Forward only: the
+    numpy backend has no gradient."""
+    import statistics
+    import numpy as np
+    import torch
+    import oracle as O
+    from oracle import numpy_path as NP
+    from libcpab_amd import Cpab
+    T3 = Cpab([3, 3], backend="pytorch", device="gpu", zero_boundary=True)
+    g = torch.Generator().manual_seed(0)
+    th3 = (torch.randn(1, T3.get_theta_dim(), generator=g) * 0.5).double().numpy()
+    img3 = np.moveaxis(torch.rand(1, 3, 100, 100, generator=g).double().numpy(), 1, -1)  # NWHC
+    mesh3 = O.uniform_meshgrid([100, 100])
+    B3 = T3.get_params().basis
+
+    def plain():
+        grid = NP.transformer(mesh3, th3, B3, [3, 3], 50)
+        return NP.interpolate(2, img3, grid, [100, 100])
+
+    plain()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plain()
+        times.append(time.perf_counter() - t0)
+    t_plain = statistics.median(times)
+    p = T.get_params()
+    rng = np.random.default_rng(seed)
+    mesh = O.uniform_meshgrid(size)
+    sel = np.sort(rng.choice(mesh.shape[1], size=npts, replace=False))
+    th0 = theta[0:1].detach().cpu().double().numpy()
+    d0 = np.moveaxis(data[0:1].detach().cpu().double().numpy(), 1, -1)
+    t0 = time.perf_counter()
+    grid = NP.transformer(mesh[:, sel], th0, p.basis, p.nc, 50)
+    NP.interpolate(2, d0, grid, [npts, 1])
+    t_img = time.perf_counter() - t0
+    return {"value": npts / t_img, "unit": "points/s (forward only)", "cores": 1, "kind": "port",
+            "sample": "the reference's numpy backend restated in numpy (FP64, nstepsolver=50, scipy expm per cell, "
+                      "NWHC interpolation; forward only -- that backend has no gradient): theta[0] x %d random "
+                      "output pixels of the 256x256 image workload, %.2f s on one thread" % (npts, t_img),
+            "plain_3x3_100x100": {"value": 1e4 / t_plain, "unit": "points/s", "seconds": t_plain,
+                                  "workload": "Cpab([3,3]), 1 theta ~ N(0, 0.5^2 I) seed 0, 100x100 meshgrid, "
+                                              "1x100x100x3 U(0,1) image, transform_grid + interpolate"}}
+
+
 def oracle_error(T, theta, pts, seed=2):
     """Max |GPU - FP64 oracle| of transform_grid at pts [ndim, n] for theta[0:1],
     and the max gradient error relative to the gradient's magnitude."""
@@ -299,16 +351,17 @@ def config_inputs(name, device, seed=0):
     raise ValueError(name)
 
 
-def run_config(args):
+def config_measure(args, workload):
     """One of the secondary configurations: batched (one launch over all
     thetas) and plain (one launch per theta) fwd+bwd points/s, and the error
-    against the FP64 oracle on a sample of the workload's grid."""
+    against the FP64 oracle on a sample of the workload's grid.  Returns the
+    JSON line as a dict."""
     import numpy as np
     import torch
     import oracle as O
     from libcpab_amd import CpabSequential
     dev = torch.device("cuda:0")
-    Ts, thetas, data, target, size, data_grad, desc = config_inputs(args.workload, dev)
+    Ts, thetas, data, target, size, data_grad, desc = config_inputs(workload, dev)
     n_theta = thetas[0].shape[0]
     points = n_theta * int(np.prod(size))
     seq = len(Ts) > 1
@@ -340,8 +393,8 @@ def run_config(args):
     pts = mesh[:, np.sort(rng.choice(mesh.shape[1], size=min(args.err_points, mesh.shape[1]), replace=False))]
     errs = [oracle_error(T, th, pts) for T, th in zip(Ts, thetas)]
     fd_errs = [fd_gradient_error(T, th, pts) for T, th in zip(Ts, thetas)]
-    line = {
-        "metric": "fwd+bwd points/s (%s); max err vs FP64 oracle" % args.workload,
+    return {
+        "metric": "fwd+bwd points/s (%s); max err vs FP64 oracle" % workload,
         "value": points / sec, "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "ms_per_step_median": med * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -355,7 +408,10 @@ def run_config(args):
         "err_sample": "theta[0] of each stage, %d grid points (finite differences: the first %d)" % (
             pts.shape[1], min(1024, pts.shape[1])),
     }
-    print(json.dumps(line), flush=True)
+
+
+def run_config(args):
+    print(json.dumps(config_measure(args, args.workload)), flush=True)
 
 
 def run_copy(args):
@@ -443,7 +499,41 @@ def ragged_problems(n, seed=0):
     return probs
 
 
-def run_ragged(args):
+def ragged_oracle_errors(b, probs):
+    """Forward / gradient error of the graph-replayed batch against the FP64
+    oracle on two whole instances: the smallest [8,8] one (tables that fill
+    the LDS) and the largest 1D one.  The forward is read from the saved
+    warped points; dL/dtheta is compared with the oracle's exact gradient of
+    the same grad_out (dL/dgrid from the FP64 interpolation backward)."""
+    import numpy as np
+    import oracle as O
+    npts = [int(np.prod(o)) for _, _, _, o in probs]
+    big = [i for i, (T, _, _, _) in enumerate(probs) if T.params.nc == [8, 8]]
+    one = [i for i, (T, _, _, _) in enumerate(probs) if T.params.ndim == 1]
+    pick = [min(big, key=lambda i: npts[i])] if big else []
+    pick += [max(one, key=lambda i: npts[i])] if one else []
+    ef, eg = 0.0, 0.0
+    for i in pick:
+        T, C, size, osz = probs[i]
+        p = T.get_params()
+        nd = p.ndim
+        A = O.expand_A(p.basis, b.theta_views[i].detach().double().cpu().numpy()[None], nd)
+        grid = O.uniform_meshgrid(osz)
+        exact = O.cf_forward(nd, p.nc, A, p.facet_planes, p.facet_nbr, grid)[0]
+        exact_t = exact.reshape((nd,) + tuple(osz[::-1])).transpose((0,) + tuple(range(nd, 0, -1))).reshape(nd, -1)
+        warped = b.saved_views[i][:nd].detach().double().cpu().numpy()
+        ef = max(ef, float(np.abs(warped - exact_t).max()))
+        data = b.data_views[i].detach().double().cpu().numpy()[None]
+        gout = b.grad_out_views[i].detach().double().cpu().numpy()[None]
+        _, ggrid = O.interpolate_backward(nd, data, exact[None], osz, gout)
+        gA, _ = O.cf_backward(nd, p.nc, A, p.facet_planes, p.facet_nbr, grid, ggrid)
+        gth = (gA.reshape(1, -1) @ p.basis)[0]
+        got = b.grad_theta_views[i].detach().double().cpu().numpy()
+        eg = max(eg, float(np.abs(got - gth).max() / np.abs(gth).max()))
+    return ef, eg, ["%d: %s %s" % (i, probs[i][0].params.nc, probs[i][3]) for i in pick]
+
+
+def ragged_measure(args):
     import torch
     from libcpab_amd.batch import CpabBatch
     dev = torch.device("cuda:0")
@@ -491,7 +581,12 @@ def run_ragged(args):
     psec = timed(step_plain, max(1, args.plain_steps), 1)
     value = b.points / sec
     log("ragged batched %.3f ms/step, plain %.3f ms/step" % (sec * 1e3, psec * 1e3))
-    line = {
+    # errors of the graph-replayed batch: one more replayed step, then the oracle
+    step_batched()
+    torch.cuda.synchronize()
+    err_fwd, err_grad, err_inst = ragged_oracle_errors(b, probs)
+    log("ragged errors vs FP64 oracle: fwd %.3g, grad %.3g (%s)" % (err_fwd, err_grad, err_inst))
+    return {
         "metric": "fwd+bwd points/s, ragged 1D/2D mix: batched (persistent, graph-captured) vs plain HIP",
         "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -502,8 +597,15 @@ def run_ragged(args):
                    "instances": b.n, "points_per_step": b.points, "ppt": b.ppt, "blocks": b.blocks},
         "plain": {"value": b.points / psec, "ms_per_step": psec * 1e3},
         "batched_over_plain": psec / sec,
+        "max_err_fwd": err_fwd,
+        "max_rel_err_grad": err_grad,
+        "err_sample": "whole instances %s: warped points of the replayed forward and dL/dtheta of the replayed "
+                      "backward against the FP64 oracle" % ", ".join(err_inst),
     }
-    print(json.dumps(line), flush=True)
+
+
+def run_ragged(args):
+    print(json.dumps(ragged_measure(args)), flush=True)
 
 
 BWD_KERNEL = "k_transform_data_bwd<2, true>"
@@ -558,6 +660,8 @@ def main():
     ap.add_argument("--workload", default="images2d", choices=["images2d", "ragged", "ts1d", "vol3d", "seq2d", "copy",
                                                                   "align"])
     ap.add_argument("--instances", type=int, default=1000)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="images2d only: skip the ragged and seq2d sub-objects of the line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this path has no multi-GPU mode (the theta batch would be split "
@@ -600,10 +704,20 @@ def main():
     achieved = algo_bytes / (bwd_ms * 1e-3) / 1e9
     log("k_transform_data_bwd call: %.3f ms; forward call %.3f ms" % (bwd_ms, fwd_ms))
 
-    res, cpu = {}, None
+    res, cpu, cpu_np = {}, None, None
     if not args.no_cpu_baseline:
         res, cpu = cpu_baseline(T, theta, data, target, size, args.cpu_points, args.err_points)
         log("cpu baseline:", cpu, res)
+        cpu_np = numpy_baseline(T, theta, data, size, args.cpu_points)
+        log("numpy baseline:", cpu_np)
+
+    extra = {}
+    if not args.no_extra:
+        keys = ("ms_per_step", "value", "unit", "config", "plain", "batched_over_plain", "max_err_fwd",
+                "max_rel_err_grad", "max_rel_err_grad_fd", "err_sample")
+        for name in ("ragged", "seq2d"):
+            r = ragged_measure(args) if name == "ragged" else config_measure(args, name)
+            extra[name] = {k: r[k] for k in keys if k in r}
 
     line = {
         "metric": METRIC,
@@ -634,7 +748,9 @@ def main():
                      "algorithmic_bytes_per_launch": algo_bytes, "launch_ms": bwd_ms},
         "valu": valu_roofline(BWD_KERNEL, bwd_ms),
         "cpu_baseline": cpu,
+        "cpu_baseline_numpy": cpu_np,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 
