@@ -219,12 +219,17 @@ _AXES = collections.OrderedDict()
 
 def meshgrid_axes(outsize, device):
     """The axis coordinates of libcpab/pytorch/functions.py:102-108, concatenated.
-    Built once per (outsize, device) and kept (read-only) for later calls."""
+    Built once per (outsize, device) and kept (read-only) for later calls.
+    Under graph capture an uncached tensor is built on the capturing stream
+    instead (no host synchronisation, which capture forbids; its memory then
+    belongs to the graph, which recomputes it on every replay)."""
     key = (tuple(int(n) for n in outsize), device.type, device.index)
     lin = _AXES.get(key)
     if lin is None:
         lin = torch.cat([torch.linspace(0, 1, int(n), device=device) for n in outsize]).contiguous()
         if lin.is_cuda:
+            if torch.cuda.is_current_stream_capturing():
+                return lin
             torch.cuda.current_stream(device).synchronize()  # usable from any stream from now on
         _AXES[key] = lin
         if len(_AXES) > 64:

@@ -75,3 +75,32 @@ def test_alignment_by_sampling_never_increases_the_misfit(cuda):
     start = torch.norm(x1 - x2).item()
     end = torch.norm(T.transform_data(x1, theta, [64, 64]) - x2).item()
     assert end <= start * (1 + 1e-6)
+
+
+def test_first_transform_data_of_an_output_size_under_graph_capture(cuda):
+    """The meshgrid axes of an output size not seen before are built inside
+    the capture (no host synchronisation, no cache entry); replays equal the
+    eager call, with eager work and synchronisations between them."""
+    T = Cpab([3, 3], backend="pytorch", device="gpu", zero_boundary=True)
+    g = torch.Generator().manual_seed(3)
+    theta = (torch.randn(2, T.get_theta_dim(), generator=g) * 0.4).to(cuda)
+    data = torch.rand(2, 3, 45, 39, generator=g).to(cuda)
+    size = [37, 41]  # used by no other test
+    with torch.no_grad():  # tables on the device, kernels loaded: eager, another size
+        T.transform_data(data, theta, [20, 20])
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.no_grad():
+        with torch.cuda.graph(graph, stream=s):
+            out = T.transform_data(data, theta, size)
+        torch.cuda.synchronize()
+        eager = T.transform_data(data, theta, size)
+        for _ in range(3):
+            out.zero_()
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager)
+            torch.randn(1000, device=cuda).sum().item()
