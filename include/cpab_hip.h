@@ -39,6 +39,7 @@ extern "C" {
 #define CPAB_ERR_HIP 4
 #define CPAB_ERR_LDS 5
 #define CPAB_ERR_UNSUPPORTED 6
+#define CPAB_ERR_SCRATCH 7
 
 /* Tessellation on the device.  planes [nC, 4, 4] f32 (n0, n1, n2, e) unit
  * outward facet planes in q = x*nc space, nbr [nC, 4] i32 cell across each
@@ -136,6 +137,20 @@ int cpab_interpolate_bwd(int ndim, const float* data, int N, int C, const int* d
 int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
                             const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
                             float* saved, void* workspace, size_t ws_bytes, hipStream_t stream);
+
+/* cpab_transform_data_fwd without a workspace: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  A_q
+ * lives in a static device array of CPAB_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends; n_theta * nC * ndim * (ndim+1) * 4 bytes beyond it
+ * return CPAB_ERR_SCRATCH.  The call allocates nothing.  All calls on one GPU
+ * share that array: calls that overlap on one GPU (concurrent streams or
+ * host threads) race.  The stream must belong to the current device.  Same
+ * results as cpab_transform_data_fwd. */
+#define CPAB_NOALLOC_SCRATCH_BYTES (4u << 20)
+int cpab_transform_data_fwd_noalloc(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                                    const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                                    float* saved, hipStream_t stream);
 
 /* Backward of cpab_transform_data_fwd: grad_data (nullable, zero-filled,
  * [n_theta, C, dsize...]) and grad_theta [n_theta, d].  saved may be NULL

@@ -50,6 +50,8 @@ _SIGNATURES = {
                                       _c_ptr, _c_ptr]),
     "cpab_transform_data_fwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr, _c_ptr,
                                          _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "cpab_transform_data_fwd_noalloc": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr,
+                                                 _c_ptr, _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr]),
     "cpab_transform_data_bwd": (_c_int, [ctypes.POINTER(CpabTess), _c_ptr, _c_int, _c_ptr, _c_int, _c_ptr, _c_ptr,
                                          _c_ptr, _c_float, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "cpab_findcell": (_c_int, [_c_int, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr]),
@@ -73,6 +75,9 @@ _SIGNATURES = {
 }
 
 EXPORTED = tuple(_SIGNATURES)
+
+# static scratch per GPU of cpab_transform_data_fwd_noalloc (CPAB_NOALLOC_SCRATCH_BYTES)
+CPAB_NOALLOC_SCRATCH_BYTES = 4 << 20
 
 _lib = None
 

@@ -18,6 +18,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "cpab_device.hpp"
 #include "cpab_numeric.hpp"
 #include "../../include/cpab_hip.h"
@@ -1640,6 +1642,53 @@ int launch_reduce(const cpab_tess_t* t, float* part, int n_theta, int tiles, flo
 }  // namespace
 
 // ===========================================================================
+// static scratch of cpab_transform_data_fwd_noalloc
+// ===========================================================================
+// A_q of the allocation-free forward lives here instead of in a workspace.
+// Each device that loads the library holds its own copy until the process
+// ends.  In host code the array's name is not a device address: kernels get
+// the address hipGetSymbolAddress returns on the device (kept per device).
+static __device__ __align__(256) float scratch[CPAB_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// Device address of `scratch` on the stream's device, looked up once per
+// device with that device current.  The stream must belong to the current
+// device (the null stream always does).
+int scratch_on_device(hipStream_t stream, float** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        (void)hipGetLastError();
+        return CPAB_ERR_HIP;
+    }
+    if (stream) {
+        hipDevice_t sdev = -1;
+        if (hipStreamGetDevice(stream, &sdev) != hipSuccess) {
+            (void)hipGetLastError();
+            return CPAB_ERR_HIP;
+        }
+        if (sdev != dev) return CPAB_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return CPAB_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CPAB_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
 // extern "C" boundary
 // ===========================================================================
 extern "C" {
@@ -1653,6 +1702,9 @@ const char* cpab_status_string(int status) {
         case CPAB_ERR_HIP: return "HIP launch failed";
         case CPAB_ERR_LDS: return "tessellation too large for the LDS budget of one block";
         case CPAB_ERR_UNSUPPORTED: return "unsupported configuration";
+        case CPAB_ERR_SCRATCH:
+            return "problem needs more than the CPAB_NOALLOC_SCRATCH_BYTES of static scratch of the allocation-free "
+                   "entry point";
         default: return "unknown status";
     }
 }
@@ -1890,9 +1942,11 @@ size_t cpab_data_workspace_bytes(const cpab_tess_t* t, int n_theta, const int* o
     return align256((size_t)n_theta * Dt * 4) + align256((size_t)n_theta * tiles * Dt * 4);
 }
 
-int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
-                            const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
-                            float* saved, void* workspace, size_t ws_bytes, hipStream_t stream) {
+// The fused forward with A_q either in the caller's workspace or (workspace
+// NULL, ws_bytes 0, from_scratch) in this device's static scratch array.
+static int data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                    const int* dsize, const int* outsize, const float* lin, float t_end, float* out, float* saved,
+                    void* workspace, size_t ws_bytes, bool from_scratch, hipStream_t stream) {
     int rc = check_tess(t);
     if (rc) return rc;
     Img im;
@@ -1902,7 +1956,15 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     if ((rc = src_out(t->ndim, outsize, s, nP))) return rc;
     if (!theta || !lin || !out || n_theta < 0) return CPAB_ERR_ARG;
     if (n_theta == 0) return CPAB_OK;
-    if (ws_bytes < cpab_data_workspace_bytes(t, n_theta, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
+    float* Aq;
+    if (from_scratch) {
+        const size_t Dt = (size_t)t->nC * t->ndim * (t->ndim + 1);
+        if ((size_t)n_theta * Dt * sizeof(float) > (size_t)CPAB_NOALLOC_SCRATCH_BYTES) return CPAB_ERR_SCRATCH;
+        if ((rc = scratch_on_device(stream, &Aq))) return rc;
+    } else {
+        if (ws_bytes < cpab_data_workspace_bytes(t, n_theta, outsize) || !workspace) return CPAB_ERR_WORKSPACE;
+        Aq = static_cast<float*>(workspace);
+    }
     Launch L;
     L.n_theta = n_theta;
     L.nP = nP;
@@ -1910,7 +1972,6 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.ty = tile_ty(t->ndim, s.g1, s.g2);
     const size_t lds = lds_tables(t->ndim, t->nC);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
-    float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     s.lin = lin;
     L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt, s.g2);
@@ -1925,6 +1986,19 @@ int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_thet
     CPAB_DISPATCH_D(t->ndim, LAUNCH_DF)
 #undef LAUNCH_DF
     return status_from_hip();
+}
+
+int cpab_transform_data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                            const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                            float* saved, void* workspace, size_t ws_bytes, hipStream_t stream) {
+    return data_fwd(t, theta, n_theta, data, C, dsize, outsize, lin, t_end, out, saved, workspace, ws_bytes, false,
+                    stream);
+}
+
+int cpab_transform_data_fwd_noalloc(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
+                                    const int* dsize, const int* outsize, const float* lin, float t_end, float* out,
+                                    float* saved, hipStream_t stream) {
+    return data_fwd(t, theta, n_theta, data, C, dsize, outsize, lin, t_end, out, saved, nullptr, 0, true, stream);
 }
 
 int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_theta, const float* data, int C,
