@@ -54,12 +54,15 @@ def _lookup2(p, nx, ny):
     return (base + off).astype(np.int64)
 
 
-def _lookup3(p, nx, ny, nz):
+def push_inside(p, nc):
+    """3D points outside [0,1]^3 moved onto the domain boundary the way the
+    3D lookup does before it looks them up (coordinates beyond the faces are
+    set to the face, or half a cell inside it for the pushed axis); points
+    inside are returned unchanged."""
     p = np.array(p, dtype=np.float64, copy=True)
-    inc = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
+    inc = 1.0 / np.asarray(nc, dtype=np.float64)
     out = np.any((p < 0.0) | (p > 1.0), axis=0)
     if out.any():
-        # push outside points onto the boundary, about the domain centre
         c = p[:, out] - 0.5
         a = np.abs(c)
         push = np.stack([0.5 * inc[0] * ((a[0] < a[1]) & (a[0] < a[2])),
@@ -68,6 +71,12 @@ def _lookup3(p, nx, ny, nz):
         beyond = a > 0.5
         c = np.where(beyond, np.copysign(0.5 - push, c), c)
         p[:, out] = c + 0.5
+    return p
+
+
+def _lookup3(p, nx, ny, nz):
+    p = push_inside(p, (nx, ny, nz))
+    inc = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
     lim = np.array([nx, ny, nz]) * inc - 1e-8
     q = np.minimum(lim[:, None], np.maximum(0.0, p))
     m = np.mod(q, inc[:, None])

@@ -213,10 +213,12 @@ def test_findcell_is_exact_except_within_1e6_of_a_facet(cuda, name):
     """cpab_findcell (FP32) against the oracle's FP64 numpy-semantics lookup:
     every mismatch lies within 1e-6 (x-space) of the region of the cell the
     GPU returned, measured with the facet table (q = x * nc space, unit
-    normals; the signed x-space distance is G / |n * nc|).  3D
-    points outside the domain (looked up after the reference's push onto the
-    boundary) must match unless the oracle's own answer changes when the
-    point moves by 1e-6 along an axis."""
+    normals; the signed x-space distance is G / |n * nc|).  3D points
+    outside the domain are looked up after the reference's push onto the
+    boundary (restated in oracle/numpy_path.py and pinned there), so for them
+    the pushed point is measured.  At most 1% of the random points (inside
+    the domain in 3D) may differ at all; the points on diagonals and grid
+    lines are chosen to sit on facets."""
     T, b = make_cpab(name)
     nc = [int(v) for v in b["nc"]]
     nd = len(nc)
@@ -230,25 +232,24 @@ def test_findcell_is_exact_except_within_1e6_of_a_facet(cuda, name):
     ref = O.findcell(nd, nc, p64)
     bad = np.flatnonzero(idx != ref)
     inside = np.all((p64 >= 0) & (p64 <= 1), axis=0)
+    if nd == 3:
+        from oracle.numpy_path import push_inside
+        p64 = push_inside(p64, nc)
     q = p64.T * np.array(nc, dtype=np.float64)
     planes, nbr = b["planes"], b["nbr"]
     for i in bad:
-        if nd == 3 and not inside[i]:
-            moved = []
-            for a in range(nd):
-                for sgn in (1.0, -1.0):
-                    pp = p64[:, i:i + 1].copy()
-                    pp[a] += sgn * 1e-6
-                    moved.append(int(O.findcell(nd, nc, pp)[0]))
-            assert any(m != ref[i] for m in moved), (i, pts[:, i], idx[i], ref[i])
-            continue
         c = idx[i]
         # signed x-space distance to each facet plane n.q + e = 0, q = nc * x
-        G = (planes[c, :, :nd] @ q[i] + planes[c, :, 3]) / np.linalg.norm(planes[c, :, :nd] * nc, axis=1)
-        G[nbr[c] == -2] = -np.inf  # domain walls bound no region
+        live = nbr[c] != -2  # 1D/2D domain walls (unused slots) bound no region
+        G = (planes[c, live, :nd] @ q[i] + planes[c, live, 3]) / np.linalg.norm(planes[c, live, :nd] * nc, axis=1)
         assert G.max() <= 1e-6, (i, pts[:, i], idx[i], ref[i], G.max())
-    print(name, "points", pts.shape[1], "mismatches", bad.size)
-    assert bad.size <= 0.01 * pts.shape[1]
+    rand = np.zeros(pts.shape[1], bool)
+    rand[:4000] = True
+    if nd == 3:
+        rand &= inside
+    frac = float((idx != ref)[rand].mean())
+    print(name, "points", pts.shape[1], "mismatches", bad.size, "of the random ones", frac)
+    assert frac <= 0.01
 
 
 @pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_2x2_nzb", "3d_2x2x2", "3d_3x3x3"])
