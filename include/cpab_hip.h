@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CPAB_ABI_VERSION 4
+#define CPAB_ABI_VERSION 5
 
 #define CPAB_OK 0
 #define CPAB_ERR_ARG 1
@@ -45,7 +45,13 @@ extern "C" {
  * outward facet planes in q = x*nc space, nbr [nC, 4] i32 cell across each
  * facet (<0: none).  Both may be NULL for ndim == 1.  Replaces the
  * (params.nc, params.basis) pair that libcpab/pytorch/transformer.py:146-159
- * ships to the device on every call. */
+ * ships to the device on every call.
+ * unfinished (nullable, device int): the closed-form integration gives every
+ * point at most 512 chunks and facet crossings (fields with |M|_F * t_end up
+ * to ~256 per cell fit); a point that uses them up with time left gets NaN
+ * outputs (point, Jacobian, so also its gradients), and each forward call
+ * (cpab_transform_fwd, the cpab_transform_data_fwd family, cpab_batch_fwd)
+ * adds the number of such points here.  The caller zeroes it. */
 typedef struct cpab_tess {
     int ndim;
     int nc[3];
@@ -54,6 +60,7 @@ typedef struct cpab_tess {
     const float* B;
     const float* planes;
     const int* nbr;
+    int* unfinished;
 } cpab_tess_t;
 
 const char* cpab_status_string(int status);

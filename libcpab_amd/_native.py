@@ -19,7 +19,7 @@ _c_ptr = ctypes.c_void_p
 class CpabTess(ctypes.Structure):
     """Mirror of cpab_tess_t."""
     _fields_ = [("ndim", _c_int), ("nc", _c_int * 3), ("nC", _c_int), ("d", _c_int),
-                ("B", _c_ptr), ("planes", _c_ptr), ("nbr", _c_ptr)]
+                ("B", _c_ptr), ("planes", _c_ptr), ("nbr", _c_ptr), ("unfinished", _c_ptr)]
 
 
 class CpabInstance(ctypes.Structure):
@@ -76,6 +76,9 @@ _SIGNATURES = {
 
 EXPORTED = tuple(_SIGNATURES)
 
+# CPAB_ABI_VERSION of include/cpab_hip.h that these signatures and structs mirror
+ABI_VERSION = 5
+
 # static scratch per GPU of cpab_transform_data_fwd_noalloc (CPAB_NOALLOC_SCRATCH_BYTES)
 CPAB_NOALLOC_SCRATCH_BYTES = 4 << 20
 
@@ -98,6 +101,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.cpab_abi_version() != ABI_VERSION:
+            raise CpabError("libcpab_hip.so has ABI version %d, this binding expects %d: rebuild it"
+                            % (L.cpab_abi_version(), ABI_VERSION))
         _lib = L
     return _lib
 

@@ -88,6 +88,7 @@ class Cpab(object):
         p.method = "closed_form"
         p.store_jacobian = True
         p.launch = "batched"
+        p.raise_unfinished = False
         if p.ndim == 1:
             p.nC = p.nc[0]
             p.params_pr_cell = 2
@@ -134,8 +135,18 @@ class Cpab(object):
     def get_basis(self):
         return self.params.basis
 
+    def unfinished_points(self, reset=True):
+        """Points that used up the closed-form integrator's chunk budget (512
+        chunks and facet crossings per point; their outputs and gradients are
+        NaN) in this transformer's forward calls since the last reset.  Reads
+        device counters (a synchronisation).  set_solver_params(
+        raise_unfinished=True) checks after every forward call instead and
+        raises CpabError."""
+        from .pytorch.hip_ops import unfinished_points
+        return unfinished_points(self.params, reset)
+
     def set_solver_params(self, nstepsolver=50, numeric_grad=False, use_slow=False, method=None,
-                          store_jacobian=None, launch=None):
+                          store_jacobian=None, launch=None, raise_unfinished=None):
         """libcpab/cpab.py:140-157.  nstepsolver and numeric_grad apply to
         method='numeric' (the reference's fixed-step scheme); the default
         method='closed_form' integrates exactly and ignores them.  use_slow
@@ -157,6 +168,8 @@ class Cpab(object):
         if launch is not None:
             assert launch in ("batched", "plain"), "launch must be 'batched' or 'plain'"
             self.params.launch = launch
+        if raise_unfinished is not None:
+            self.params.raise_unfinished = bool(raise_unfinished)
 
     # -- sampling -----------------------------------------------------------
     def uniform_meshgrid(self, n_points):

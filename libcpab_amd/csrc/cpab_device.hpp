@@ -51,7 +51,8 @@ constexpr int KS = 7;           // series terms
 constexpr float RHO = 0.5f;     // chunk bound |M|_F * h
 constexpr float GROW = 1.6488f;   // e^RHO: speed growth bound over a chunk
 constexpr int NSAMP = 4;        // samples per chunk for root isolation
-constexpr int MAX_ITERS = 512;  // chunks + crossings per point (termination bound)
+constexpr int MAX_ITERS = 512;  // chunks + crossings per point (termination bound; a point that
+                                // has time left after them is unfinished: NaN, and counted)
 constexpr float ZERO_HOP = 1e-7f;
 constexpr int MAX_ZERO_HOPS = 8;
 constexpr int NREC = 2;         // chunks per point the forward records for the backward (2D/3D)
@@ -508,6 +509,13 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
         vabs += t;
     }
     const float tol1 = 1e-6f * vabs;
+    // A velocity inside that band is rounding noise: the point sits on a
+    // fixed point of the field at FP32 resolution (e.g. a domain corner under
+    // zero_boundary) and moves by at most e^RHO |v0| h over the chunk.  It
+    // crosses no facet in this chunk (otherwise the noise could carry it
+    // back and forth across a facet through the fixed point, a hop per
+    // chunk, until the chunk budget ran out).
+    const bool still = vn <= tol1;
     // Bounds on the chunk (|n| = 1, |M| <= |M|_F, |M|_F h <= RHO, |cf_k| <=
     // |M|^(k-1) |v0| / k!):  |G''(s)| <= K = |M|_F |v0| e^RHO and
     // |G''(s) - 2 cf2 - 6 cf3 s| <= K4 = |M|_F^3 |v0| h^2 e^RHO / 2.
@@ -530,7 +538,7 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
         cglo[f] = 0.f;
         cghi[f] = 0.f;
         const int nbf = tab.nb[c * NF + f];
-        if (nbf < 0) continue;
+        if (nbf < 0 || still) continue;
         const float4 P = tab.pl[c * NF + f];
         const float n[3] = {P.x, P.y, P.z};
         float g0 = P.w, g1 = 0.f, g2 = 0.f;
@@ -957,24 +965,37 @@ CPAB_HD __forceinline__ void identity(float (&J)[D][D]) {
 }
 
 // Integrate one point for time T (pass 1 if WITHJ); with LOG its first NREC
-// chunks go into lg for the backward to replay.
+// chunks go into lg for the backward to replay.  Returns false when the point
+// used up MAX_ITERS with time left: q (and J) are then NaN rather than a
+// point short of its end.
 template <int D, bool WITHJ, bool LOG = false, class Log = ChunkLog>
-CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+CPAB_HD __forceinline__ bool integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
                                        float (&J)[D][D], Log& lg) {
     PointState<D> st;
     init_point<D>(st, q, c0, T);
     if constexpr (WITHJ) identity<D>(J);
     lg.code = 0;
     while (point_active<D>(st)) step_point<D, WITHJ, LOG, Log>(st, tab, nx, tol0, J, lg);
+    const bool done = !(st.T > 0.f);
+    const float nan = __builtin_nanf("");
 #pragma unroll
-    for (int a = 0; a < D; ++a) q[a] = st.q[a];
+    for (int a = 0; a < D; ++a) q[a] = done ? st.q[a] : nan;
+    if constexpr (WITHJ) {
+        if (!done) {
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int k = 0; k < D; ++k) J[r][k] = nan;
+        }
+    }
+    return done;
 }
 
 template <int D, bool WITHJ>
-CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+CPAB_HD __forceinline__ bool integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
                                        float (&J)[D][D]) {
     ChunkLog lg;
-    integrate<D, WITHJ, false>(q, c0, T, tab, nx, tol0, J, lg);
+    return integrate<D, WITHJ, false>(q, c0, T, tab, nx, tol0, J, lg);
 }
 
 // ---------------------------------------------------------------------------

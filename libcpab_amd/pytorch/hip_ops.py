@@ -34,6 +34,8 @@ class DeviceTess:
         self.planes = torch.as_tensor(np.ascontiguousarray(params.facet_planes), dtype=torch.float32,
                                       device=device).contiguous()
         self.nbr = torch.as_tensor(np.ascontiguousarray(params.facet_nbr), dtype=torch.int32, device=device).contiguous()
+        # points that used up the chunk budget in a forward (their outputs are NaN)
+        self.unfinished = torch.zeros(1, dtype=torch.int32, device=device)
         t = N.CpabTess()
         t.ndim = self.ndim
         t.nc = N.int3(self.nc)
@@ -42,6 +44,7 @@ class DeviceTess:
         t.B = self.B.data_ptr()
         t.planes = self.planes.data_ptr()
         t.nbr = self.nbr.data_ptr()
+        t.unfinished = self.unfinished.data_ptr()
         self.struct = t
         self.ref = ctypes.byref(t)
 
@@ -65,6 +68,26 @@ def device_tess(params, device):
     if key not in cache:
         cache[key] = DeviceTess(params, torch.device("cuda", key[1]))
     return cache[key]
+
+
+def unfinished_points(params, reset=True):
+    """Points that used up the chunk budget (NaN outputs) in the forward calls
+    of this tessellation since the last reset, summed over devices.  Reads
+    device counters: synchronises with the devices' current streams."""
+    total = 0
+    for dev in getattr(params, "_device_tess", {}).values():
+        total += int(dev.unfinished.item())
+        if reset:
+            dev.unfinished.zero_()
+    return total
+
+
+def _check_unfinished(params):
+    if getattr(params, "raise_unfinished", False):
+        n = unfinished_points(params)
+        if n:
+            raise N.CpabError("%d point(s) used up the integrator's chunk budget (512 chunks and crossings per "
+                              "point); their outputs are NaN -- the field is too strong for t_end" % n)
 
 
 def _stream(device):
@@ -124,6 +147,7 @@ class TransformGrid(torch.autograd.Function):
         shape = _int_arr(grid_shape) if grid_shape is not None else None
         N.check(N.lib().cpab_transform_fwd(dev.ref, _ptr(theta), n_theta, _ptr(points), nP, per_theta, shape,
                                            float(t_end), _ptr(out), _ptr(jac), _ptr(ws), nb, _stream(theta.device)))
+        _check_unfinished(params)
         ctx.params = params
         ctx.t_end = t_end
         ctx.per_theta = per_theta
@@ -263,6 +287,7 @@ class TransformData(torch.autograd.Function):
         fn = N.lib().cpab_transform_data_fwd_plain if plain else N.lib().cpab_transform_data_fwd
         N.check(fn(dev.ref, _ptr(theta), n_theta, _ptr(data), C, _int_arr(data.shape[2:]), _int_arr(outsize),
                    _ptr(lin), float(t_end), _ptr(out), _ptr(saved), _ptr(ws), nb, _stream(theta.device)))
+        _check_unfinished(params)
         ctx.params = params
         ctx.outsize = tuple(outsize)
         ctx.t_end = t_end

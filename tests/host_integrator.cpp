@@ -10,10 +10,12 @@
 
 using namespace cpab;
 
+// returns the number of points that used up the chunk budget (NaN outputs)
 template <int D>
-static void run(const int* nc, int nC, const float* Aq, const float* planes, const int* nbr, const float* pts,
-                int nP, int n_theta, int per_theta, float t_end, float tol0, float* out, float* jac) {
+static int run(const int* nc, int nC, const float* Aq, const float* planes, const int* nbr, const float* pts,
+               int nP, int n_theta, int per_theta, float t_end, float tol0, float* out, float* jac) {
     constexpr int P = D * (D + 1);
+    int unfinished = 0;
     float* inv = (float*)malloc(sizeof(float) * nC);
     for (int t = 0; t < n_theta; ++t) {
         const float* A = Aq + (size_t)t * nC * P;
@@ -36,7 +38,7 @@ static void run(const int* nc, int nC, const float* Aq, const float* planes, con
                 q[a] = x[a] * (float)nc[a];
             }
             const int c0 = find_cell<D>(x, nc);
-            integrate<D, true>(q, c0, t_end, tab, nc[0], tol0, J);
+            if (!integrate<D, true>(q, c0, t_end, tab, nc[0], tol0, J)) ++unfinished;
             for (int a = 0; a < D; ++a) out[((size_t)t * D + a) * nP + p] = q[a] / (float)nc[a];
             if (jac)
                 for (int r = 0; r < D; ++r)
@@ -44,15 +46,15 @@ static void run(const int* nc, int nC, const float* Aq, const float* planes, con
         }
     }
     free(inv);
+    return unfinished;
 }
 
 extern "C" int host_transform(int ndim, const int* nc, int nC, const float* Aq, const float* planes, const int* nbr,
                               const float* pts, int nP, int n_theta, int per_theta, float t_end, float tol0,
                               float* out, float* jac) {
-    if (ndim == 1) run<1>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
-    else if (ndim == 2) run<2>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
-    else run<3>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
-    return 0;
+    if (ndim == 1) return run<1>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
+    if (ndim == 2) return run<2>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
+    return run<3>(nc, nC, Aq, planes, nbr, pts, nP, n_theta, per_theta, t_end, tol0, out, jac);
 }
 
 extern "C" int host_findcell(int ndim, const int* nc, const float* pts, int nP, int* idx) {

@@ -279,3 +279,37 @@ def test_host_replay_of_the_chunk_log_is_bit_exact(nc, zb, scale):
                                             ctypes.c_float(1.0), ctypes.c_float(2e-6 * max(nc)), ctypes.byref(logged))
         assert bad == 0, bad
         assert logged.value >= grid.shape[1]
+
+
+@pytest.mark.parametrize("k,finished", [(20.0, True), (400.0, False)])
+def test_host_chunk_budget_exhaustion_is_nan_and_counted(k, finished):
+    """Every cell of a [2,2] tessellation contracts towards its own centroid,
+    q' = -k (q - c) in q-space, and a point started near the centroid stays
+    in the cell.  Chunks obey |M|_F h <= 1/2, so T = 1 takes 2 sqrt(2) k of
+    them: k = 20 finishes (q(T) = c + (q0 - c) e^-k), k = 400 would need about
+    1130 > 512 and must come back as NaN -- point and Jacobian -- and be
+    counted, not returned short of its end."""
+    nc = [2, 2]
+    t = make_tessellation(nc, zero_boundary=True)
+    planes, nbr = t.facet_table()
+    cq = t.get_cell_centers() * np.array(nc)  # centroids in q-space
+    Aq = np.zeros((t.nC, 2, 3), np.float32)
+    Aq[:, 0, 0] = Aq[:, 1, 1] = -k
+    Aq[:, :, 2] = k * cq
+    pts = (cq + 0.01).T / np.array(nc)[:, None]  # one point per cell, x-space
+    out = np.zeros((1, 2, t.nC), np.float32)
+    jac = np.zeros((1, 4, t.nC), np.float32)
+    ncarr = np.array(nc + [1], np.int32)
+    n_bad = _lib().host_transform(2, _ptr(ncarr), t.nC, _ptr(np.ascontiguousarray(Aq.reshape(1, -1))),
+                                  _ptr(np.ascontiguousarray(planes, np.float32)),
+                                  _ptr(np.ascontiguousarray(nbr, np.int32)),
+                                  _ptr(np.ascontiguousarray(pts, np.float32)), t.nC, 1, 0, ctypes.c_float(1.0),
+                                  ctypes.c_float(2e-6 * 2), _ptr(out), _ptr(jac))
+    if finished:
+        assert n_bad == 0
+        want = (cq.T + (pts * np.array(nc)[:, None] - cq.T) * np.exp(-k)) / np.array(nc)[:, None]
+        assert np.abs(out[0] - want).max() < 1e-6
+        assert np.isfinite(jac).all()
+    else:
+        assert n_bad == t.nC
+        assert np.isnan(out).all() and np.isnan(jac).all()
