@@ -505,8 +505,9 @@ def ragged_oracle_errors(b, probs):
     the LDS) and the largest 1D one.  The forward is read from the saved
     warped points; dL/dtheta is compared with the oracle's exact adjoint of
     the integration applied to dL/dgrid of the same grad_out, taken by the
-    FP64 interpolation backward at the GPU's warped points (the interpolant
-    of a noise image changes slope at every pixel edge)."""
+    FP64 interpolation backward at the GPU's warped points with the sample
+    coordinates formed in FP32 like the GPU's (the interpolant of a noise
+    image changes slope at every pixel edge)."""
     import numpy as np
     import oracle as O
     npts = [int(np.prod(o)) for _, _, _, o in probs]
@@ -528,7 +529,7 @@ def ragged_oracle_errors(b, probs):
         data = b.data_views[i].detach().double().cpu().numpy()[None]
         gout = b.grad_out_views[i].detach().double().cpu().numpy()[None]
         w_pts = warped.reshape((nd,) + tuple(osz)).transpose((0,) + tuple(range(nd, 0, -1))).reshape(1, nd, -1)
-        _, ggrid = O.interpolate_backward(nd, data, w_pts, osz, gout)
+        _, ggrid = O.interpolate_backward(nd, data, w_pts, osz, gout, index32=True)
         gA, _ = O.cf_backward(nd, p.nc, A, p.facet_planes, p.facet_nbr, grid, ggrid)
         gth = (gA.reshape(1, -1) @ p.basis)[0]
         got = b.grad_theta_views[i].detach().double().cpu().numpy()

@@ -39,7 +39,13 @@ struct Geo {
     const float4* pl;
     const int* nb;
     float tol0;
+    int* unfinished;  // nullable: count of points that used up the chunk budget
 };
+
+// a point that used up the chunk budget (its outputs are NaN) is counted
+__device__ __forceinline__ void count_unfinished(const Geo& g, bool done) {
+    if (!done && g.unfinished) atomicAdd(g.unfinished, 1);
+}
 
 struct Src {  // where points come from
     const float* pts;  // explicit [ndim, nP] or [n_theta, ndim, nP]
@@ -290,6 +296,10 @@ __device__ __forceinline__ void pass2(bool valid, const float (&q0)[D], int c0, 
 // interpolation helpers (libcpab/pytorch/interpolation.py semantics)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void axis_corners(float x, int size, int& i0, int& i1, float& fd) {
+    // the rounded FP32 product, as the reference's float32 tensors form it:
+    // no contraction of x * (size - 1) into fd's subtraction (an FMA would
+    // keep the unrounded product, up to half an ulp of x * (size - 1) away)
+#pragma clang fp contract(off)
     const float xs = x * (float)(size - 1);
     float f0 = floorf(xs);
     f0 = fminf(fmaxf(f0, -1.f), (float)size);
@@ -458,7 +468,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
         float* jo = jac + (size_t)th * jac_rows(D) * L.nP;
         LogSink lg;
         if constexpr (WITHJ && D > 1) lg.init(jo + (size_t)D * D * L.nP, L.nP, p);
-        integrate<D, WITHJ, WITHJ && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
+        count_unfinished(g, integrate<D, WITHJ, WITHJ && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg));
         float* o = out + (size_t)th * D * L.nP;
 #pragma unroll
         for (int a = 0; a < D; ++a) o[(size_t)a * L.nP + p] = q[a] * rnc[a];
@@ -565,7 +575,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
         // with saved state (2D/3D) the chunk durations go straight to their rows
         LogSink lg;
         if constexpr (SAVE && D > 1) lg.init(sv + (size_t)(D + D * D) * L.nP, L.nP, t);
-        integrate<D, SAVE, SAVE && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
+        count_unfinished(g, integrate<D, SAVE, SAVE && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg));
         float y[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) y[a] = q[a] * rnc[a];
@@ -1189,6 +1199,7 @@ __device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, flo
     g.nC = I->tess.nC;
     g.pl = reinterpret_cast<const float4*>(I->tess.planes);
     g.nb = I->tess.nbr;
+    g.unfinished = I->tess.unfinished;
     int m = g.nc0 > g.nc1 ? g.nc0 : g.nc1;
     m = m > g.nc2 ? m : g.nc2;
     g.tol0 = 2e-6f * (float)m;
@@ -1546,6 +1557,7 @@ Geo make_geo(const cpab_tess_t* t) {
     g.nC = t->nC;
     g.pl = reinterpret_cast<const float4*>(t->planes);
     g.nb = t->nbr;
+    g.unfinished = t->unfinished;
     int m = g.nc0 > g.nc1 ? g.nc0 : g.nc1;
     m = m > g.nc2 ? m : g.nc2;
     g.tol0 = 2e-6f * (float)(m > 1 ? m : 1);
@@ -1625,15 +1637,19 @@ int launch_reduce(const cpab_tess_t* t, float* part, int n_theta, int tiles, flo
             return status_from_hip();
         }
     }
+    // k_reduce_cols keeps [16 waves, cw] floats in LDS: at most cols_fit columns per slice
+    const int nw = RBLOCK / 64;
+    const int cols_fit = (int)(LDS_LIMIT / ((size_t)nw * sizeof(float)));
+    if (nslice < (Dt + cols_fit - 1) / cols_fit) nslice = (Dt + cols_fit - 1) / cols_fit;
     const int cw = (Dt + nslice - 1) / nslice;
     nslice = (Dt + cw - 1) / cw;
-    const int nw = RBLOCK / 64;
-    hipLaunchKernelGGL(k_reduce_cols, dim3(n_theta, nslice), dim3(nw * 64), (size_t)nw * cw * sizeof(float), st,
-                       part, tiles, Dt, cw, t->ndim, g.nc0, g.nc1, g.nc2, gA);
-    // B^T: as many waves (up to 16) as the LDS holds Dt + nw * d floats for
+    // B^T: as many waves (up to 16) as the LDS holds Dt + nw * d floats for;
+    // checked before the first launch
     int nwb = RBLOCK / 64;
     while (nwb > 1 && (size_t)(Dt + nwb * t->d) * sizeof(float) > LDS_LIMIT) nwb >>= 1;
     if ((size_t)(Dt + nwb * t->d) * sizeof(float) > LDS_LIMIT) return CPAB_ERR_LDS;
+    hipLaunchKernelGGL(k_reduce_cols, dim3(n_theta, nslice), dim3(nw * 64), (size_t)nw * cw * sizeof(float), st,
+                       part, tiles, Dt, cw, t->ndim, g.nc0, g.nc1, g.nc2, gA);
     hipLaunchKernelGGL(k_reduce_bt, dim3(n_theta), dim3(nwb * 64), (size_t)(Dt + nwb * t->d) * sizeof(float), st,
                        part, tiles, Dt, t->B, t->d, gtheta);
     return status_from_hip();

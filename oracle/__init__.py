@@ -256,8 +256,15 @@ def uniform_meshgrid(n_points, dtype=np.float64):
     return np.stack([m.ravel() for m in mesh[::-1]]).astype(dtype)
 
 
-def _corners(x, size):
-    x = x * (size - 1)
+def _corners(x, size, index32=False):
+    """Corner indices and fraction of coordinate x on an axis of `size`
+    samples.  index32: x * (size - 1) is formed in FP32, as the reference's
+    float32 tensors (libcpab/pytorch/interpolation.py:75-82) and the GPU do,
+    so that a point within FP32 rounding of a sample picks the same cell."""
+    if index32:
+        x = (np.asarray(x, dtype=np.float32) * np.float32(size - 1)).astype(np.float64)
+    else:
+        x = x * (size - 1)
     x0 = np.floor(x).astype(np.int64)
     x1 = x0 + 1
     x0 = np.clip(x0, 0, size - 1)
@@ -265,23 +272,24 @@ def _corners(x, size):
     return x0, x1, x - x0
 
 
-def interpolate(ndim, data, grid, outsize):
+def interpolate(ndim, data, grid, outsize, index32=False):
     """Linear interpolation of data [N, C, *size] at grid [N, ndim, nP]
-    (libcpab/pytorch/interpolation.py:18-171).  Returns [N, C, *outsize]."""
+    (libcpab/pytorch/interpolation.py:18-171).  Returns [N, C, *outsize].
+    index32: sample coordinates x * (size - 1) in FP32 (see _corners)."""
     data = np.asarray(data, dtype=np.float64)
     grid = np.asarray(grid, dtype=np.float64)
     N, C = data.shape[:2]
     size = data.shape[2:]
     out = np.zeros((N, C, grid.shape[-1]))
     for n in range(N):
-        idx, w = _weights(ndim, grid[n], size)
+        idx, w = _weights(ndim, grid[n], size, index32)
         for ci, wi in zip(idx, w):
             out[n] += data[n][(slice(None),) + tuple(ci)] * wi
     return _to_outsize(out, outsize)
 
 
-def _weights(ndim, g, size):
-    cs = [_corners(g[a], size[a]) for a in range(ndim)]
+def _weights(ndim, g, size, index32=False):
+    cs = [_corners(g[a], size[a], index32) for a in range(ndim)]
     idx, w = [], []
     for bits in np.ndindex(*([2] * ndim)):
         ci = [cs[a][1] if bits[a] else cs[a][0] for a in range(ndim)]
@@ -305,9 +313,10 @@ def _from_outsize(g, outsize):
     return g.transpose((0, 1) + tuple(range(nd + 1, 1, -1))).reshape(N, C, -1)
 
 
-def interpolate_backward(ndim, data, grid, outsize, grad_out):
+def interpolate_backward(ndim, data, grid, outsize, grad_out, index32=False):
     """Gradients of <grad_out, interpolate(data, grid)> w.r.t. data and grid,
-    as torch autograd gives them for libcpab/pytorch/interpolation.py."""
+    as torch autograd gives them for libcpab/pytorch/interpolation.py.
+    index32: sample coordinates in FP32 (see _corners)."""
     data = np.asarray(data, dtype=np.float64)
     grid = np.asarray(grid, dtype=np.float64)
     go = _from_outsize(np.asarray(grad_out, dtype=np.float64), outsize)
@@ -316,7 +325,7 @@ def interpolate_backward(ndim, data, grid, outsize, grad_out):
     gdata = np.zeros_like(data)
     ggrid = np.zeros_like(grid)
     for n in range(N):
-        cs = [_corners(grid[n, a], size[a]) for a in range(ndim)]
+        cs = [_corners(grid[n, a], size[a], index32) for a in range(ndim)]
         for bits in np.ndindex(*([2] * ndim)):
             ci = tuple(cs[a][1] if bits[a] else cs[a][0] for a in range(ndim))
             wi = np.ones(grid.shape[-1])

@@ -548,3 +548,24 @@ def test_vectorfield_is_the_time_derivative_of_the_flow(name, scale):
     scale_v = max(np.abs(v).max(), 1.0)
     assert errs[1] < 1e-4 * scale_v
     assert errs[1] < 0.2 * errs[0] + 1e-9  # shrinks with eps
+
+
+def test_fp32_sample_coordinates_pick_the_float32_cell():
+    """index32 forms x * (size - 1) in FP32, as torch float32 tensors do:
+    equal to the FP64 interpolation away from pixel edges, and for a float32
+    point just below sample 9 of 600 whose FP32 product rounds onto 9 it takes
+    the slope of cell [9, 10] (the FP64 product stays in [8, 9])."""
+    rng = np.random.default_rng(3)
+    data = rng.uniform(size=(1, 2, 600))
+    grid = rng.uniform(0, 1, size=(1, 1, 300))
+    a = O.interpolate(1, data, grid, [300])
+    b = O.interpolate(1, data, grid, [300], index32=True)
+    assert np.abs(a - b).max() < 1e-4
+    x = np.nextafter(np.float32(9 / 599), np.float32(0))
+    assert np.float32(x) * np.float32(599) == 9.0 and float(x) * 599 < 9.0
+    pt = np.array([[[float(x)]]])
+    go = np.ones((1, 2, 1))
+    _, g64 = O.interpolate_backward(1, data, pt, [1], go)
+    _, g32 = O.interpolate_backward(1, data, pt, [1], go, index32=True)
+    slope = lambda i: (data[0, :, i + 1] - data[0, :, i]).sum() * 599
+    assert abs(g64[0, 0, 0] - slope(8)) < 1e-9 and abs(g32[0, 0, 0] - slope(9)) < 1e-9
