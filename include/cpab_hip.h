@@ -269,6 +269,26 @@ int cpab_ref_forward(float* newpoints, const float* points, const float* trels, 
 int cpab_ref_backward(float* grad, const float* points, const float* As, const float* Bs, int nstepsolver,
                       const int* nc, int n_theta, int d, int ndim, int nP, int nC, int broadcast, hipStream_t stream);
 
+/* ---- use_slow: the reference's pure-torch stepping path ------------------ */
+
+/* out [n_theta, ndim, nP] = nstepsolver steps x <- Trel[cell(x)] [x; 1] of
+ * each point under its theta's trels [n_theta, nC, ndim, ndim+1] (the first
+ * ndim rows of expm(A~ / nstepsolver)), the cell from the torch lookup
+ * (libcpab/pytorch/findcellidx.py; numpy rules, 3D clamp 1 - 1e-4).  Replaces
+ * CPAB_transformer_slow, libcpab/pytorch/transformer.py:90-128. */
+int cpab_slow_forward(const float* points, const float* trels, int nstepsolver, const int* nc, int ndim, int nP,
+                      int n_theta, int per_theta_grid, float* out, hipStream_t stream);
+
+/* Exact gradient of <grad_out, cpab_slow_forward(points, trels)> w.r.t. the
+ * points (grad_points [n_theta, ndim, nP], nullable) and the trels
+ * (grad_trels, same shape as trels, zero-filled by the caller, accumulated
+ * with atomics).  trels_inv [n_theta, nC, ndim, ndim] holds the inverse of
+ * each Trel's linear part.  The reference gets this gradient from torch
+ * autograd through the loop of libcpab/pytorch/transformer.py:117-121. */
+int cpab_slow_backward(const float* points, const float* trels, const float* trels_inv, int nstepsolver,
+                       const int* nc, int ndim, int nP, int n_theta, int per_theta_grid, const float* grad_out,
+                       float* grad_points, float* grad_trels, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,9 @@ _SIGNATURES = {
     "cpab_fill_zero": (_c_int, [_c_ptr, _c_size, _c_ptr]),
     "cpab_expm_affine": (_c_int, [_c_ptr, _c_int, _c_int, _c_float, _c_ptr, _c_ptr]),
     "cpab_ref_forward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_int, _c_ptr]),
+    "cpab_slow_forward": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_int, _c_ptr, _c_ptr]),
+    "cpab_slow_backward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_int, _c_ptr,
+                                    _c_ptr, _c_ptr, _c_ptr]),
     "cpab_ref_backward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_int, _c_int, _c_int, _c_int,
                                    _c_int, _c_int, _c_ptr]),
 }

@@ -147,12 +147,19 @@ class Cpab(object):
 
     def set_solver_params(self, nstepsolver=50, numeric_grad=False, use_slow=False, method=None,
                           store_jacobian=None, launch=None, raise_unfinished=None):
-        """libcpab/cpab.py:140-157.  nstepsolver and numeric_grad apply to
-        method='numeric' (the reference's fixed-step scheme); the default
-        method='closed_form' integrates exactly and ignores them.  use_slow
-        (the reference's pure-torch version of the same scheme,
-        libcpab/pytorch/transformer.py:72-128) is accepted and checked but
-        selects the same HIP kernels: there is no torch fallback path."""
+        """libcpab/cpab.py:140-157.  use_slow=True selects the reference's
+        slow path (libcpab/pytorch/transformer.py:90-128): nstepsolver steps
+        of expm(A~/nstepsolver) with the torch cell lookup, differentiated
+        exactly w.r.t. theta and the points (HIP kernels for the steps and
+        their adjoint, torch autograd through matrix_exp).  Otherwise
+        method='numeric' runs the reference's compiled fixed-step path
+        (nstepsolver steps; numeric_grad=True for its finite-difference
+        gradient) and the default method='closed_form' integrates exactly,
+        ignoring nstepsolver and numeric_grad.  Beyond the reference:
+        store_jacobian (closed form: keep the forward's Jacobians for the
+        backward), launch ('batched' or 'plain', one launch sequence per
+        theta) and raise_unfinished (check every forward for points that
+        used up the chunk budget and raise CpabError)."""
         assert nstepsolver > 0, "nstepsolver must be a positive number"
         assert type(nstepsolver) == int, "nstepsolver must be integer"
         assert type(numeric_grad) == bool, "numeric_grad must be bool"
@@ -249,7 +256,7 @@ class Cpab(object):
         self._check_device(data)
         self._check_type(theta)
         self._check_device(theta)
-        if getattr(self.params, "method", "closed_form") == "closed_form":
+        if getattr(self.params, "method", "closed_form") == "closed_form" and not self.params.use_slow:
             return self.backend.transform_data(data, theta, self.params, outsize)
         grid = self.uniform_meshgrid(outsize)
         grid_t = self.transform_grid(grid, theta)

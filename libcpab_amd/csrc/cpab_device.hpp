@@ -794,7 +794,10 @@ CPAB_HD __forceinline__ int find_cell2(float x, float y, int nx, int ny) {
     return base + ((1.f - fx < fy) ? 1 : 0);
 }
 
-CPAB_HD __forceinline__ int find_cell3(float x, float y, float z, int nx, int ny, int nz) {
+// top: largest coordinate the lookup keeps (libcpab/numpy/findcellidx.py
+// clamps to 1 - 1e-8, which is 1 in FP32; the torch lookup of the reference's
+// use_slow path clamps to 1 - 1e-4, libcpab/pytorch/findcellidx.py:112-114)
+CPAB_HD __forceinline__ int find_cell3(float x, float y, float z, int nx, int ny, int nz, float top = 1.f) {
     float p[3] = {x, y, z};
     const int n[3] = {nx, ny, nz};
     const bool outside = x < 0.f || x > 1.f || y < 0.f || y > 1.f || z < 0.f || z > 1.f;
@@ -816,7 +819,7 @@ CPAB_HD __forceinline__ int find_cell3(float x, float y, float z, int nx, int ny
     float loc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float q = fminf(fmaxf(p[a], 0.f), 1.f) * (float)n[a];
+        const float q = fminf(fmaxf(p[a], 0.f), top) * (float)n[a];
         ijk[a] = clampi((int)floorf(q), 0, n[a] - 1);
         loc[a] = q - (float)ijk[a];
     }

@@ -1530,6 +1530,151 @@ __global__ __launch_bounds__(BLOCK) void k_ref_backward(float* __restrict__ grad
 }
 
 // ---------------------------------------------------------------------------
+// use_slow: the reference's pure-torch stepping (libcpab/pytorch/transformer.py:
+// 90-128) -- nstep applications of Trel = expm(A~/nstep) of the cell the torch
+// lookup finds -- with the exact adjoint of that discrete map as its gradient
+// (what torch autograd gives the reference), w.r.t. the points and the Trels
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ int slow_cell(const float (&x)[D], const int* nc) {
+    if constexpr (D == 3) return find_cell3(x[0], x[1], x[2], nc[0], nc[1], nc[2], 1.f - 1e-4f);
+    else return find_cell<D>(x, nc);
+}
+
+// x <- T[:, :D] x + T[:, D] for a [D, D+1] row-major affine map
+template <int D>
+__device__ __forceinline__ void slow_step(const float* __restrict__ T, float (&x)[D]) {
+    float y[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        float acc = T[r * (D + 1) + D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc = fmaf(T[r * (D + 1) + k], x[k], acc);
+        y[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) x[r] = y[r];
+}
+
+// grid (ceil(nP/BLOCK), n_theta); trels [n_theta, nC, D, D+1] staged in LDS
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_slow_forward(const float* __restrict__ points, const float* __restrict__ trels,
+                                                        int nstep, int nc0, int nc1, int nc2, int nC, int nP,
+                                                        int per_theta, float* __restrict__ out) {
+    extern __shared__ float sT[];
+    constexpr int P = D * (D + 1);
+    const int th = blockIdx.y;
+    for (int k = threadIdx.x; k < nC * P; k += BLOCK) sT[k] = trels[(size_t)th * nC * P + k];
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= nP) return;
+    const int nc[3] = {nc0, nc1, nc2};
+    const float* pts = points + (per_theta ? (size_t)th * D * nP : 0);
+    float x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = pts[(size_t)a * nP + i];
+    for (int n = 0; n < nstep; ++n) slow_step<D>(sT + P * slow_cell<D>(x, nc), x);
+#pragma unroll
+    for (int a = 0; a < D; ++a) out[((size_t)th * D + a) * nP + i] = x[a];
+}
+
+// Exact adjoint of k_slow_forward for x_{k+1} = M_k x_k + b_k, L(x_n):
+// lambda_k = M_k^T lambda_{k+1}, dL/dM_k += lambda_{k+1} x_k^T, dL/db_k +=
+// lambda_{k+1}, dL/dx_0 = lambda_0.  No trajectory storage: pass 1 carries
+// J = M_{n-1}..M_0 (lambda_0 = J^T g), pass 2 steps forward again carrying
+// lambda_{k+1} = M_k^-T lambda_k (tinv holds M^-1 per cell) and adds each
+// step's outer products to the block's LDS accumulator, which goes to
+// gtrels (zero-filled by the caller) by global float atomics.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_slow_backward(const float* __restrict__ points,
+                                                         const float* __restrict__ trels, const float* __restrict__ tinv,
+                                                         int nstep, int nc0, int nc1, int nc2, int nC, int nP,
+                                                         int per_theta, const float* __restrict__ gout,
+                                                         float* __restrict__ gpts, float* __restrict__ gtrels) {
+    extern __shared__ float smem[];
+    constexpr int P = D * (D + 1);
+    const int th = blockIdx.y;
+    float* sT = smem;
+    float* sI = sT + nC * P;
+    float* sG = sI + nC * D * D;
+    for (int k = threadIdx.x; k < nC * P; k += BLOCK) {
+        sT[k] = trels[(size_t)th * nC * P + k];
+        sG[k] = 0.f;
+    }
+    for (int k = threadIdx.x; k < nC * D * D; k += BLOCK) sI[k] = tinv[(size_t)th * nC * D * D + k];
+    __syncthreads();
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < nP) {
+        const int nc[3] = {nc0, nc1, nc2};
+        const float* pts = points + (per_theta ? (size_t)th * D * nP : 0);
+        float x0[D], x[D], J[D][D], lam[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            x0[a] = pts[(size_t)a * nP + i];
+            x[a] = x0[a];
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int k = 0; k < D; ++k) J[r][k] = (r == k) ? 1.f : 0.f;
+        for (int n = 0; n < nstep; ++n) {
+            const float* T = sT + P * slow_cell<D>(x, nc);
+            float Jn[D][D];
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) acc = fmaf(T[r * (D + 1) + a], J[a][k], acc);
+                    Jn[r][k] = acc;
+                }
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int k = 0; k < D; ++k) J[r][k] = Jn[r][k];
+            slow_step<D>(T, x);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            float acc = 0.f;
+#pragma unroll
+            for (int r = 0; r < D; ++r) acc = fmaf(J[r][k], gout[((size_t)th * D + r) * nP + i], acc);
+            lam[k] = acc;
+        }
+        if (gpts) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) gpts[((size_t)th * D + k) * nP + i] = lam[k];
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[a] = x0[a];
+        for (int n = 0; n < nstep; ++n) {
+            const int c = slow_cell<D>(x, nc);
+            const float* Mi = sI + D * D * c;
+            float ln[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                float acc = 0.f;
+#pragma unroll
+                for (int r = 0; r < D; ++r) acc = fmaf(Mi[r * D + k], lam[r], acc);
+                ln[k] = acc;
+            }
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                lam[r] = ln[r];
+#pragma unroll
+                for (int k = 0; k < D; ++k) atomicAdd(sG + P * c + r * (D + 1) + k, lam[r] * x[k]);
+                atomicAdd(sG + P * c + r * (D + 1) + D, lam[r]);
+            }
+            slow_step<D>(sT + P * c, x);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nC * P; k += BLOCK)
+        if (sG[k] != 0.f) atomicAdd(gtrels + (size_t)th * nC * P + k, sG[k]);
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 int cells_of(int ndim, const int* nc) {
@@ -2360,6 +2505,51 @@ int cpab_ref_backward(float* grad, const float* points, const float* As, const f
                        n_theta, nC, nP, broadcast ? 1 : 0);
     CPAB_DISPATCH_D(ndim, LAUNCH_RB)
 #undef LAUNCH_RB
+    return status_from_hip();
+}
+
+int cpab_slow_forward(const float* points, const float* trels, int nstepsolver, const int* nc, int ndim, int nP,
+                      int n_theta, int per_theta_grid, float* out, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    if (!nc || nstepsolver < 0 || nP < 0 || n_theta < 0) return CPAB_ERR_ARG;
+    if (nP == 0 || n_theta == 0) return CPAB_OK;
+    if (!points || !trels || !out) return CPAB_ERR_ARG;
+    for (int a = 0; a < ndim; ++a)
+        if (nc[a] < 1) return CPAB_ERR_ARG;
+    if (n_theta > 65535) return CPAB_ERR_UNSUPPORTED;
+    const int n0 = nc[0], n1 = ndim > 1 ? nc[1] : 1, n2 = ndim > 2 ? nc[2] : 1;
+    const int nC = cells_of(ndim, nc);
+    const size_t lds = (size_t)nC * ndim * (ndim + 1) * 4;
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    dim3 g((nP + BLOCK - 1) / BLOCK, n_theta);
+#define LAUNCH_SF(DD)                                                                                                \
+    hipLaunchKernelGGL((k_slow_forward<DD>), g, dim3(BLOCK), lds, stream, points, trels, nstepsolver, n0, n1, n2, nC, \
+                       nP, per_theta_grid ? 1 : 0, out);
+    CPAB_DISPATCH_D(ndim, LAUNCH_SF)
+#undef LAUNCH_SF
+    return status_from_hip();
+}
+
+int cpab_slow_backward(const float* points, const float* trels, const float* trels_inv, int nstepsolver,
+                       const int* nc, int ndim, int nP, int n_theta, int per_theta_grid, const float* grad_out,
+                       float* grad_points, float* grad_trels, hipStream_t stream) {
+    if (ndim < 1 || ndim > 3) return CPAB_ERR_NDIM;
+    if (!nc || nstepsolver < 0 || nP < 0 || n_theta < 0) return CPAB_ERR_ARG;
+    if (nP == 0 || n_theta == 0) return CPAB_OK;
+    if (!points || !trels || !trels_inv || !grad_out || !grad_trels) return CPAB_ERR_ARG;
+    for (int a = 0; a < ndim; ++a)
+        if (nc[a] < 1) return CPAB_ERR_ARG;
+    if (n_theta > 65535) return CPAB_ERR_UNSUPPORTED;
+    const int n0 = nc[0], n1 = ndim > 1 ? nc[1] : 1, n2 = ndim > 2 ? nc[2] : 1;
+    const int nC = cells_of(ndim, nc);
+    const size_t lds = (size_t)nC * (2 * ndim * (ndim + 1) + ndim * ndim) * 4;
+    if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
+    dim3 g((nP + BLOCK - 1) / BLOCK, n_theta);
+#define LAUNCH_SB(DD)                                                                                                  \
+    hipLaunchKernelGGL((k_slow_backward<DD>), g, dim3(BLOCK), lds, stream, points, trels, trels_inv, nstepsolver, n0, \
+                       n1, n2, nC, nP, per_theta_grid ? 1 : 0, grad_out, grad_points, grad_trels);
+    CPAB_DISPATCH_D(ndim, LAUNCH_SB)
+#undef LAUNCH_SB
     return status_from_hip();
 }
 

@@ -114,7 +114,13 @@ def uniform_meshgrid(ndim, domain_min, domain_max, n_points, device="cpu"):
 
 def transformer(points, theta, params):
     """Integrate points under theta (replaces CPAB_transformer,
-    libcpab/pytorch/transformer.py:72-87).  Returns [n_theta, ndim, nP]."""
+    libcpab/pytorch/transformer.py:72-87).  Returns [n_theta, ndim, nP].
+    use_slow selects the reference's pure-torch stepping path (its autograd
+    gradient, w.r.t. theta and the points); otherwise method='closed_form'
+    (exact integration) or 'numeric' (the reference's compiled fast path)."""
+    if getattr(params, "use_slow", False):
+        from .slow import transformer_slow
+        return transformer_slow(points, theta, params)
     method = getattr(params, "method", "closed_form")
     if method != "closed_form":
         from .numeric import transformer_numeric

@@ -55,7 +55,7 @@ class CpabSequential(object):
         c._check_device(grid)
         c._check_type(theta)
         c._check_device(theta)
-        if getattr(c.params, "method", "closed_form") != "closed_form":
+        if getattr(c.params, "method", "closed_form") != "closed_form" or c.params.use_slow:
             return c.transform_grid(grid, theta)
         return TransformGrid.apply(grid, theta, c.params, 1.0, False, grid_shape)
 

@@ -74,10 +74,10 @@ def push_inside(p, nc):
     return p
 
 
-def _lookup3(p, nx, ny, nz):
+def _lookup3(p, nx, ny, nz, margin=1e-8):
     p = push_inside(p, (nx, ny, nz))
     inc = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
-    lim = np.array([nx, ny, nz]) * inc - 1e-8
+    lim = np.array([nx, ny, nz]) * inc - margin
     q = np.minimum(lim[:, None], np.maximum(0.0, p))
     m = np.mod(q, inc[:, None])
     ijk = np.stack([_round_min(n - 1, (q[a] - m[a]) / inc[a]) for a, n in enumerate((nx, ny, nz))])
@@ -92,10 +92,14 @@ def _lookup3(p, nx, ny, nz):
     return (base + off).astype(np.int64)
 
 
-def findcellidx(ndim, p, nc):
-    """Cell index of each point of p [ndim, nP] (numpy-backend rules)."""
+def findcellidx(ndim, p, nc, margin3=1e-8):
+    """Cell index of each point of p [ndim, nP] (numpy-backend rules; the 3D
+    upper clamp 1 - margin3 is 1 - 1e-4 in the torch backend's lookup,
+    libcpab/pytorch/findcellidx.py:112-114)."""
     p = np.asarray(p, dtype=np.float64)
-    return (_lookup1, _lookup2, _lookup3)[ndim - 1](p, *nc)
+    if ndim == 3:
+        return _lookup3(p, *nc, margin=margin3)
+    return (_lookup1, _lookup2)[ndim - 1](p, *nc)
 
 
 def trels(As, nstep):

@@ -569,3 +569,73 @@ def test_fp32_sample_coordinates_pick_the_float32_cell():
     _, g32 = O.interpolate_backward(1, data, pt, [1], go, index32=True)
     slope = lambda i: (data[0, :, i + 1] - data[0, :, i]).sum() * 599
     assert abs(g64[0, 0, 0] - slope(8)) < 1e-9 and abs(g32[0, 0, 0] - slope(9)) < 1e-9
+
+
+# --------------------------------------------------------------------------
+# the use_slow path (FP64 restatement and its adjoint)
+# --------------------------------------------------------------------------
+def _slow_setup(name, seed, scale=0.6, nstep=20):
+    from oracle import slow_path as S
+    b, nc, ndim, A, theta = _setup(name, 2, scale, seed=seed)
+    tr = O.trels(A, nstep)
+    lo, hi = (0.02, 0.98) if bool(b["zero_boundary"]) else (-0.1, 1.1)
+    pts = np.random.default_rng(seed).uniform(lo, hi, size=(ndim, 60))
+    return S, b, nc, ndim, tr, pts
+
+
+@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_2x2_nzb", "3d_2x2x2"])
+def test_slow_path_forward_is_the_numpy_stepping_with_the_torch_lookup(name):
+    """Away from the domain's upper faces (where the torch 3D lookup clamps at
+    1 - 1e-4) the slow path's steps are the numpy backend's."""
+    from oracle import numpy_path as NP
+    S, b, nc, ndim, tr, pts = _slow_setup(name, 30)
+    got = S.slow_forward(pts, tr, nc, 20)
+    _, _, _, A, theta = _setup(name, 2, 0.6, seed=30)
+    ref = NP.transformer(pts, theta, b["B"], nc, 20)
+    assert np.abs(got - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["1d_10", "2d_3x3", "2d_2x2_nzb", "3d_2x2x2"])
+def test_slow_path_adjoint_matches_central_differences(name):
+    """The reverse sweep against FP64 central differences of slow_forward
+    w.r.t. every Trel entry and every point coordinate.  Points whose cell
+    sequence changes under one of the 1e-7 steps (the map is only piecewise
+    smooth) are left out of the functional; most points stay."""
+    S, b, nc, ndim, tr, pts = _slow_setup(name, 31)
+    eps = 1e-7
+    _, _, cs0 = S.slow_forward(pts, tr, nc, 20, keep=True)
+    n_theta, nP = tr.shape[0], pts.shape[1]
+    steps = []
+    for idx in np.ndindex(*tr.shape):
+        for sgn in (1.0, -1.0):
+            t_ = tr.copy()
+            t_[idx] += sgn * eps
+            steps.append((t_, pts))
+    for a in range(ndim):
+        for sgn in (1.0, -1.0):
+            p_ = pts.copy()
+            p_[a] += sgn * eps
+            steps.append((tr, p_))
+    stable = np.ones(n_theta * nP, bool)
+    for t_, p_ in steps:
+        _, _, cs = S.slow_forward(p_, t_, nc, 20, keep=True)
+        for a, b_ in zip(cs, cs0):
+            stable &= a == b_
+    assert stable.mean() > 0.8
+    w = np.random.default_rng(32).standard_normal((n_theta, ndim, nP))
+    w *= stable.reshape(n_theta, 1, nP)
+    gT, gp = S.slow_backward(pts, tr, nc, 20, w)
+    fd = np.zeros_like(tr)
+    for idx in np.ndindex(*tr.shape):
+        tp, tm = tr.copy(), tr.copy()
+        tp[idx] += eps
+        tm[idx] -= eps
+        fd[idx] = ((S.slow_forward(pts, tp, nc, 20) - S.slow_forward(pts, tm, nc, 20)) * w).sum() / (2 * eps)
+    assert np.abs(gT - fd).max() <= 1e-6 * max(np.abs(fd).max(), 1.0)
+    fdp = np.zeros((n_theta, ndim, nP))
+    for a in range(ndim):
+        pp, pm = pts.copy(), pts.copy()
+        pp[a] += eps
+        pm[a] -= eps
+        fdp[:, a] = ((S.slow_forward(pp, tr, nc, 20) - S.slow_forward(pm, tr, nc, 20)) * w).sum(1) / (2 * eps)
+    assert np.abs(gp - fdp).max() <= 1e-6 * max(np.abs(fdp).max(), 1.0)
