@@ -968,37 +968,52 @@ CPAB_HD __forceinline__ void identity(float (&J)[D][D]) {
 }
 
 // Integrate one point for time T (pass 1 if WITHJ); with LOG its first NREC
-// chunks go into lg for the backward to replay.  Returns false when the point
-// used up MAX_ITERS with time left: q (and J) are then NaN rather than a
-// point short of its end.
+// chunks go into lg for the backward to replay.  A point that uses up
+// MAX_ITERS with time left ends as NaN (q and J) rather than short of its end,
+// and adds 1 to *unfinished (nullable).  The check sits in the rare branch
+// only, so the common path carries no extra live state.
 template <int D, bool WITHJ, bool LOG = false, class Log = ChunkLog>
-CPAB_HD __forceinline__ bool integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
-                                       float (&J)[D][D], Log& lg) {
+CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+                                       float (&J)[D][D], Log& lg, int* unfinished = nullptr) {
     PointState<D> st;
     init_point<D>(st, q, c0, T);
     if constexpr (WITHJ) identity<D>(J);
     lg.code = 0;
-    while (point_active<D>(st)) step_point<D, WITHJ, LOG, Log>(st, tab, nx, tol0, J, lg);
-    const bool done = !(st.T > 0.f);
-    const float nan = __builtin_nanf("");
+    while (st.T > 0.f) {
+        step_point<D, WITHJ, LOG, Log>(st, tab, nx, tol0, J, lg);
+        if (st.iters >= MAX_ITERS && st.T > 0.f) {
+            const float nan = __builtin_nanf("");
+            st.T = 0.f;
 #pragma unroll
-    for (int a = 0; a < D; ++a) q[a] = done ? st.q[a] : nan;
-    if constexpr (WITHJ) {
-        if (!done) {
+            for (int a = 0; a < D; ++a) st.q[a] = nan;
+            if constexpr (WITHJ) {
 #pragma unroll
-            for (int r = 0; r < D; ++r)
+                for (int r = 0; r < D; ++r)
 #pragma unroll
-                for (int k = 0; k < D; ++k) J[r][k] = nan;
+                    for (int k = 0; k < D; ++k) J[r][k] = nan;
+            }
+            if (unfinished) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                // the operand is made opaque so that its register is set here,
+                // on the rare path, and not hoisted out of the loop
+                int one = 1;
+                asm volatile("" : "+v"(one));
+                atomicAdd(unfinished, one);
+#else
+                ++*unfinished;
+#endif
+            }
         }
     }
-    return done;
+#pragma unroll
+    for (int a = 0; a < D; ++a) q[a] = st.q[a];
 }
 
 template <int D, bool WITHJ>
-CPAB_HD __forceinline__ bool integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
-                                       float (&J)[D][D]) {
+CPAB_HD __forceinline__ void integrate(float (&q)[D], int c0, float T, const CellTable<D>& tab, int nx, float tol0,
+                                       float (&J)[D][D], int* unfinished = nullptr) {
     ChunkLog lg;
-    return integrate<D, WITHJ, false>(q, c0, T, tab, nx, tol0, J, lg);
+    integrate<D, WITHJ, false>(q, c0, T, tab, nx, tol0, J, lg, unfinished);
 }
 
 // ---------------------------------------------------------------------------

@@ -42,10 +42,6 @@ struct Geo {
     int* unfinished;  // nullable: count of points that used up the chunk budget
 };
 
-// a point that used up the chunk budget (its outputs are NaN) is counted
-__device__ __forceinline__ void count_unfinished(const Geo& g, bool done) {
-    if (!done && g.unfinished) atomicAdd(g.unfinished, 1);
-}
 
 struct Src {  // where points come from
     const float* pts;  // explicit [ndim, nP] or [n_theta, ndim, nP]
@@ -468,7 +464,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
         float* jo = jac + (size_t)th * jac_rows(D) * L.nP;
         LogSink lg;
         if constexpr (WITHJ && D > 1) lg.init(jo + (size_t)D * D * L.nP, L.nP, p);
-        count_unfinished(g, integrate<D, WITHJ, WITHJ && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg));
+        integrate<D, WITHJ, WITHJ && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg, g.unfinished);
         float* o = out + (size_t)th * D * L.nP;
 #pragma unroll
         for (int a = 0; a < D; ++a) o[(size_t)a * L.nP + p] = q[a] * rnc[a];
@@ -559,6 +555,9 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
     const float rnc[3] = {1.0f / (float)g.nc0, 1.0f / (float)g.nc1, 1.0f / (float)g.nc2};  // q -> x
     const int goff[3] = {0, s.g0, s.g0 + s.g1};
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
+    // points that used up the chunk budget (NaN) in this wave, counted once
+    // at the end of the tile: a wave-uniform sum, no per-lane state
+    int wave_bad = 0;
     for (int r = 0; r < L.ppt; ++r) {
         const int t = tile_t<D>(L, s, tile, r);
         if (t < 0) continue;
@@ -575,7 +574,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
         // with saved state (2D/3D) the chunk durations go straight to their rows
         LogSink lg;
         if constexpr (SAVE && D > 1) lg.init(sv + (size_t)(D + D * D) * L.nP, L.nP, t);
-        count_unfinished(g, integrate<D, SAVE, SAVE && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg));
+        integrate<D, SAVE, SAVE && (D > 1), LogSink>(q, c0, L.t_end, tab, g.nc0, g.tol0, J, lg);
         float y[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) y[a] = q[a] * rnc[a];
@@ -594,7 +593,9 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
         corners<D>(y, im, i0, i1, fd);
         for (int c = 0; c < im.C; ++c)
             out[(size_t)c * L.nP + t] = interp_value<D>(data + (size_t)c * dstride, im, i0, i1, fd);
+        wave_bad += __popcll(__ballot(y[0] != y[0]));
     }
+    if (wave_bad && g.unfinished && __lane_id() == 0) atomicAdd(g.unfinished, wave_bad);
 }
 
 template <int D, bool SAVE>

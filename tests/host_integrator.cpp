@@ -38,7 +38,7 @@ static int run(const int* nc, int nC, const float* Aq, const float* planes, cons
                 q[a] = x[a] * (float)nc[a];
             }
             const int c0 = find_cell<D>(x, nc);
-            if (!integrate<D, true>(q, c0, t_end, tab, nc[0], tol0, J)) ++unfinished;
+            integrate<D, true>(q, c0, t_end, tab, nc[0], tol0, J, &unfinished);
             for (int a = 0; a < D; ++a) out[((size_t)t * D + a) * nP + p] = q[a] / (float)nc[a];
             if (jac)
                 for (int r = 0; r < D; ++r)
