@@ -613,6 +613,7 @@ def run_ragged(args):
 
 
 BWD_KERNEL = "k_transform_data_bwd<2, true>"
+FWD_KERNEL = "k_transform_data<2, true>"
 TRAFFIC_FILE = "profiles/images2d_k_transform_data.traffic.json"
 
 
@@ -640,14 +641,21 @@ def valu_roofline(kernel, launch_ms):
     """Issue-rate roofline of `kernel`, the bound the fused kernels actually
     meet: its VALU wave-instructions per launch (SQ_INSTS_VALU from the
     committed counter summary of this workload) over the launch time measured
-    in this run, against the nominal issue peak."""
-    n = _profiled(VALU_FILE, kernel)
-    if n is None:
+    in this run, against the nominal issue peak; with the profiled fraction of
+    lanes active per VALU instruction (SQ_THREAD_CYCLES_VALU / (64
+    SQ_ACTIVE_INST_VALU)), the lane fraction (issue x lanes), and SALU per VALU
+    instruction."""
+    prof = _profiled(VALU_FILE, kernel)
+    if prof is None:
         return None
+    n = prof["valu_instr"]
     rate = n / (launch_ms * 1e-3)
+    frac = rate / VALU_ISSUE_PEAK
     return {"kernel": kernel.split("<")[0], "bound": "valu-issue", "achieved": rate / 1e9,
-            "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s", "frac": rate / VALU_ISSUE_PEAK,
-            "instr_per_launch": n, "instr_source": VALU_FILE, "launch_ms": launch_ms}
+            "peak": VALU_ISSUE_PEAK / 1e9, "unit": "G wave-instr/s", "frac": frac,
+            "lanes_active": prof["lanes_active"], "lane_frac": frac * prof["lanes_active"],
+            "salu_per_valu": prof["salu_per_valu"], "instr_per_launch": n, "instr_source": VALU_FILE,
+            "launch_ms": launch_ms}
 
 
 def main():
@@ -751,6 +759,7 @@ def main():
                      "traffic_source": TRAFFIC_FILE,
                      "algorithmic_bytes_per_launch": algo_bytes, "launch_ms": bwd_ms},
         "valu": valu_roofline(BWD_KERNEL, bwd_ms),
+        "valu_fwd": valu_roofline(FWD_KERNEL, fwd_ms),
         "cpu_baseline": cpu,
         "cpu_baseline_numpy": cpu_np,
     }
