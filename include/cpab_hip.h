@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CPAB_ABI_VERSION 5
+#define CPAB_ABI_VERSION 6
 
 #define CPAB_OK 0
 #define CPAB_ERR_ARG 1
@@ -41,11 +41,16 @@ extern "C" {
 #define CPAB_ERR_UNSUPPORTED 6
 #define CPAB_ERR_SCRATCH 7
 
-/* Tessellation on the device.  planes [nC, 4, 4] f32 (n0, n1, n2, e) unit
- * outward facet planes in q = x*nc space, nbr [nC, 4] i32 cell across each
- * facet (<0: none).  Both may be NULL for ndim == 1.  Replaces the
- * (params.nc, params.basis) pair that libcpab/pytorch/transformer.py:146-159
- * ships to the device on every call.
+/* Tessellation on the device.  planes [nC + nV, 4, 4] f32 (n0, n1, n2, e)
+ * unit outward facet planes in q = x*nc space, nbr [nC + nV, 4] i32 cell
+ * across each facet (-1: wall, -2: unused slot).  Both may be NULL for
+ * ndim == 1.  nV virtual cells follow the nC simplices (3D without zero
+ * boundary: the convex regions outside the domain); virtual cell nC + v uses
+ * the affine map of simplex amap[v] (device int [nV]), the simplex the
+ * reference's lookup gives its points after pushing them onto the boundary
+ * (libcpab/numpy/findcellidx.py:84-109).  Replaces the (params.nc,
+ * params.basis) pair that libcpab/pytorch/transformer.py:146-159 ships to the
+ * device on every call.
  * unfinished (nullable, device int): the closed-form integration gives every
  * point at most 512 chunks and facet crossings (fields with |M|_F * t_end up
  * to ~256 per cell fit); a point that uses them up with time left gets NaN
@@ -61,6 +66,8 @@ typedef struct cpab_tess {
     const float* planes;
     const int* nbr;
     int* unfinished;
+    int nV;
+    const int* amap;
 } cpab_tess_t;
 
 const char* cpab_status_string(int status);

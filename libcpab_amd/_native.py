@@ -19,7 +19,8 @@ _c_ptr = ctypes.c_void_p
 class CpabTess(ctypes.Structure):
     """Mirror of cpab_tess_t."""
     _fields_ = [("ndim", _c_int), ("nc", _c_int * 3), ("nC", _c_int), ("d", _c_int),
-                ("B", _c_ptr), ("planes", _c_ptr), ("nbr", _c_ptr), ("unfinished", _c_ptr)]
+                ("B", _c_ptr), ("planes", _c_ptr), ("nbr", _c_ptr), ("unfinished", _c_ptr), ("nV", _c_int),
+                ("amap", _c_ptr)]
 
 
 class CpabInstance(ctypes.Structure):
@@ -80,7 +81,7 @@ _SIGNATURES = {
 EXPORTED = tuple(_SIGNATURES)
 
 # CPAB_ABI_VERSION of include/cpab_hip.h that these signatures and structs mirror
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # static scratch per GPU of cpab_transform_data_fwd_noalloc (CPAB_NOALLOC_SCRATCH_BYTES)
 CPAB_NOALLOC_SCRATCH_BYTES = 4 << 20

@@ -128,12 +128,15 @@ class _TessBase:
     def facet_table(self):
         """Per-cell facets for boundary-hit integration.
 
-        Returns (planes [nC, NF_MAX, 4] float64, nbr [nC, NF_MAX] int32).
-        plane = (n0, n1, n2, e) with G(q) = n . q + e, G <= 0 inside the cell
-        region and |n|_2 = 1 in q-space.  nbr >= 0 is the cell across the
-        facet, -1 a wall (no crossing: zero-velocity domain boundary), -2 an
-        unused slot.
+        Returns (planes [nC + nV, NF_MAX, 4] float64, nbr [nC + nV, NF_MAX]
+        int32).  plane = (n0, n1, n2, e) with G(q) = n . q + e, G <= 0 inside
+        the cell region and |n|_2 = 1 in q-space.  nbr >= 0 is the cell across
+        the facet, -1 a wall (no crossing: zero-velocity domain boundary), -2
+        an unused slot.  nV virtual cells (3D without zero boundary: the
+        regions outside the domain) follow the nC simplices; self.amap [nV]
+        holds the simplex whose affine map each of them uses.
         """
+        self.amap = np.zeros(0, dtype=np.int32)
         return _simplex_facets(self)
 
 
@@ -182,15 +185,52 @@ def _simplex_facets(tess):
             per_cell[c1][f1][2] = c0
     if ndim in (1, 2):
         per_cell = _extend_outside(tess, per_cell)
-    planes = np.zeros((nC, NF_MAX, 4))
-    nbr = np.full((nC, NF_MAX), -2, dtype=np.int32)
+    vplanes = vnbr = None
+    if ndim == 3 and not tess.zero_boundary:
+        vplanes, vnbr, tess.amap, per_cell = _outside_regions_3d(tess, per_cell, qv)
+    nV = 0 if vplanes is None else vplanes.shape[0]
+    planes = np.zeros((nC + nV, NF_MAX, 4))
+    nbr = np.full((nC + nV, NF_MAX), -2, dtype=np.int32)
     for c, fl in enumerate(per_cell):
         assert len(fl) <= NF_MAX
         for f, (n, e, nb, _) in enumerate(fl):
             planes[c, f, :ndim] = n
             planes[c, f, 3] = e
             nbr[c, f] = nb
+    if nV:
+        planes[nC:] = vplanes
+        nbr[nC:] = vnbr
     return planes, nbr
+
+
+def _outside_regions_3d(tess, per_cell, qv):
+    """Virtual cells outside [0,1]^3 (libcpab_amd.core.exterior3d), planes
+    moved to q-space; the simplices' domain faces lead to them instead of
+    being walls."""
+    from .exterior3d import build_regions
+    xplanes, vnbr, amap, face_nbr, _ = build_regions(tess)
+    nc = np.asarray(tess.nc, dtype=np.float64)
+    vplanes = np.zeros_like(xplanes)
+    used = vnbr != -2
+    # n . x + e with x = q / nc: (n / nc) . q + e, then unit normal in q
+    nq = xplanes[:, :, :3] / nc
+    norm = np.linalg.norm(nq, axis=2)
+    norm[~used] = 1.0
+    vplanes[:, :, :3] = np.where(used[:, :, None], nq / norm[:, :, None], 0.0)
+    vplanes[:, :, 3] = np.where(used, xplanes[:, :, 3] / norm, 0.0)
+    out = []
+    for c, fl in enumerate(per_cell):
+        fl2 = []
+        for n, e, nb, k in fl:
+            if nb < 0:
+                # a domain face: all three vertices share a coordinate 0 or nc
+                V = np.array(k, dtype=np.float64) / 2.0
+                a = [ax for ax in range(3) if np.allclose(V[:, ax], V[0, ax])][0]
+                s = 0 if np.isclose(V[0, a], 0.0) else 1
+                nb = int(face_nbr[(c, a, s)])
+            fl2.append([n, e, nb, k])
+        out.append(fl2)
+    return vplanes, vnbr.astype(np.int32), np.asarray(amap, dtype=np.int32), out
 
 
 def _extend_outside(tess, per_cell):

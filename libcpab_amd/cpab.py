@@ -41,10 +41,12 @@ def load_or_build_basis(nc, zero_boundary, volume_perservation, override=False):
     path = os.path.join(d, basis_file_name(nc, zero_boundary, volume_perservation) + ".npz") if d else None
     if path and os.path.exists(path) and not override:
         with np.load(path, allow_pickle=False) as f:
-            return {k: f[k] for k in f.files}
+            cached = {k: f[k] for k in f.files}
+        if "amap" in cached:
+            return cached
     tess = make_tessellation(nc, zero_boundary=zero_boundary, volume_perservation=volume_perservation)
     planes, nbr = tess.facet_table()
-    out = {"L": tess.L, "B": tess.B, "planes": planes, "nbr": nbr,
+    out = {"L": tess.L, "B": tess.B, "planes": planes, "nbr": nbr, "amap": tess.amap,
            "centers": tess.get_cell_centers()}
     if path:
         tmp = path + ".%d.tmp.npz" % os.getpid()
@@ -98,9 +100,6 @@ class Cpab(object):
         else:
             p.nC = 5 * int(np.prod(p.nc))
             p.params_pr_cell = 12
-        if p.ndim == 3 and not zero_boundary:
-            raise NotImplementedError("3D tessellations with zero_boundary=False are not supported by the "
-                                      "closed-form integrator of this build")
         geo = load_or_build_basis(p.nc, zero_boundary, volume_perservation, override)
         if basis is not None:
             basis = np.asarray(basis, dtype=np.float64)
@@ -110,6 +109,8 @@ class Cpab(object):
         p.basis = geo["B"]
         p.facet_planes = geo["planes"]
         p.facet_nbr = geo["nbr"]
+        # 3D without zero boundary: the simplex each region outside the domain uses
+        p.facet_amap = np.asarray(geo["amap"], dtype=np.int32)
         p.D, p.d = p.basis.shape
         self._centers = geo["centers"]
         self.params = p

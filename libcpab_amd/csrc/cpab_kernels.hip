@@ -40,6 +40,8 @@ struct Geo {
     const int* nb;
     float tol0;
     int* unfinished;  // nullable: count of points that used up the chunk budget
+    int nV;           // virtual cells after the nC simplices (3D outside the domain)
+    const int* amap;  // [nV] simplex whose map each virtual cell uses
 };
 
 
@@ -183,25 +185,30 @@ __device__ __forceinline__ CellTable<D> table_view(float* smem, int nC) {
     return t;
 }
 
-// stage theta's q-space affine parameters, their inverse norms and the facet table
+// stage theta's q-space affine parameters, their inverse norms and the facet
+// table; a virtual cell gets a copy of the parameters of the simplex it uses
 template <int D>
 __device__ __forceinline__ CellTable<D> stage_tables(float* smem, const float* Aq, const Geo& g) {
     constexpr int P = D * (D + 1);
     const int nC = g.nC;
-    CellTable<D> t = table_view<D>(smem, nC);
+    const int nCt = nC + g.nV;
+    CellTable<D> t = table_view<D>(smem, nCt);
     float* sA = const_cast<float*>(t.A);
     float* sInv = const_cast<float*>(t.inv_n);
     float4* sPl = const_cast<float4*>(t.pl);
     int* sNb = const_cast<int*>(t.nb);
     for (int i = threadIdx.x; i < nC * P; i += BLOCK) sA[i] = Aq[i];
+    if constexpr (D == 3) {
+        for (int i = threadIdx.x; i < g.nV * P; i += BLOCK) sA[nC * P + i] = Aq[g.amap[i / P] * P + i % P];
+    }
     if constexpr (D > 1) {
-        for (int i = threadIdx.x; i < nC * NF; i += BLOCK) {
+        for (int i = threadIdx.x; i < nCt * NF; i += BLOCK) {
             sPl[i] = g.pl[i];
             sNb[i] = g.nb[i];
         }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < nC; c += BLOCK) {
+    for (int c = threadIdx.x; c < nCt; c += BLOCK) {
         float s = 0.f;
 #pragma unroll
         for (int r = 0; r < D; ++r)
@@ -226,6 +233,44 @@ __host__ __device__ constexpr int jac_rows(int D) { return D * D + (D > 1 ? NREC
 template <int D>
 __host__ __device__ constexpr size_t table_floats(int nC) {
     return (size_t)nC * D * (D + 1) + ((nC + 3) & ~3) + (D > 1 ? (size_t)nC * NF * 4 + (size_t)nC * NF : 0);
+}
+
+// Start cell of a point: the lookup, except that a 3D point outside the
+// domain of a tessellation with virtual cells starts in the region that
+// contains it -- the virtual cell whose largest facet value is the smallest
+// (its simplex is then the one the lookup returns for the pushed point).
+template <int D>
+__device__ __forceinline__ int start_cell(const float (&x)[D], const int* nc, const CellTable<D>& tab, const Geo& g) {
+    int c = find_cell<D>(x, nc);
+    if constexpr (D == 3) {
+        if (g.nV > 0 && (x[0] < 0.f || x[0] > 1.f || x[1] < 0.f || x[1] > 1.f || x[2] < 0.f || x[2] > 1.f)) {
+            const float q[3] = {x[0] * (float)nc[0], x[1] * (float)nc[1], x[2] * (float)nc[2]};
+            float best = INFINITY;
+            for (int v = g.nC; v < g.nC + g.nV; ++v) {
+                float worst = -INFINITY;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    if (tab.nb[v * NF + f] == -2) continue;
+                    const float4 P = tab.pl[v * NF + f];
+                    worst = fmaxf(worst, fmaf(P.x, q[0], fmaf(P.y, q[1], fmaf(P.z, q[2], P.w))));
+                }
+                if (worst < best) {
+                    best = worst;
+                    c = v;
+                }
+            }
+        }
+    }
+    return c;
+}
+
+// The simplex whose dL/dA a chunk in `cell` adds to (-1 stays -1)
+template <int D>
+__device__ __forceinline__ int grad_cell(int cell, const Geo& g) {
+    if constexpr (D == 3) {
+        if (cell >= g.nC) return g.amap[cell - g.nC];
+    }
+    return cell;
 }
 
 // Pass 2 over one point per lane (wave-convergent); accumulates into acc.
@@ -284,7 +329,7 @@ __device__ __forceinline__ void pass2(bool valid, const float (&q0)[D], int c0, 
                 if (ch.facet >= 0) saltation_adj<D>(tab, ch, mu);
             }
         }
-        wave_accumulate<P>(acc, cell, G);
+        wave_accumulate<P>(acc, grad_cell<D>(cell, g), G);
     }
 }
 
@@ -459,7 +504,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform(Launch L, Geo g, Src s, con
             x[a] = pts[(size_t)a * L.nP + p];
             q[a] = x[a] * (float)nc[a];
         }
-        const int c0 = find_cell<D>(x, nc);
+        const int c0 = start_cell<D>(x, nc, tab, g);
         float J[D][D];
         float* jo = jac + (size_t)th * jac_rows(D) * L.nP;
         LogSink lg;
@@ -504,7 +549,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_cells(Launch L, Geo g, Src 
             q[a] = x[a] * (float)nc[a];
         }
         PointState<D> st;
-        init_point<D>(st, q, find_cell<D>(x, nc), L.t_end);
+        init_point<D>(st, q, start_cell<D>(x, nc, tab, g), L.t_end);
         int* out = cells + ((size_t)th * L.nP + p) * max_segs;
         int n = 0;
         float seg_t = 0.f;
@@ -569,7 +614,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
             x[a] = s.lin[goff[a] + ix[a]];
             q[a] = x[a] * (float)nc[a];
         }
-        const int c0 = find_cell<D>(x, nc);
+        const int c0 = start_cell<D>(x, nc, tab, g);
         float J[D][D];
         // with saved state (2D/3D) the chunk durations go straight to their rows
         LogSink lg;
@@ -635,7 +680,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
     const int th = lb / L.tiles, tile = lb % L.tiles;
     if (th >= L.n_theta) return;
     const int nC = g.nC;
-    float* acc = smem + table_floats<D>(nC);
+    float* acc = smem + table_floats<D>(nC + g.nV);
     for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
     float* wacc = acc + (threadIdx.x >> 6) * nC * P;
@@ -661,7 +706,7 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
                 x[a] = pts[(size_t)a * L.nP + p];
                 q0[a] = x[a] * (float)nc[a];
             }
-            c0 = find_cell<D>(x, nc);
+            c0 = start_cell<D>(x, nc, tab, g);
             float J[D][D];
             if constexpr (HAVEJ) {
                 const float* jo = jac + (size_t)th * jac_rows(D) * L.nP;
@@ -807,7 +852,7 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
                 x[a] = s.lin[goff[a] + ix[a]];
                 q0[a] = x[a] * (float)nc[a];
             }
-            c0 = find_cell<D>(x, nc);
+            c0 = start_cell<D>(x, nc, tab, g);
             float J[D][D], y[D];
             if constexpr (HAVES) {
 #pragma unroll
@@ -926,7 +971,7 @@ __global__ __launch_bounds__(BLOCK, (D == 3 && HAVES) ? 4 : 1) void k_transform_
     const int th = lb / L.tiles, tile = lb % L.tiles;
     if (th >= L.n_theta) return;
     const int nC = g.nC;
-    float* acc = smem + table_floats<D>(nC);
+    float* acc = smem + table_floats<D>(nC + g.nV);
     for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
@@ -1201,6 +1246,8 @@ __device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, flo
     g.pl = reinterpret_cast<const float4*>(I->tess.planes);
     g.nb = I->tess.nbr;
     g.unfinished = I->tess.unfinished;
+    g.nV = I->tess.nV;
+    g.amap = I->tess.amap;
     int m = g.nc0 > g.nc1 ? g.nc0 : g.nc1;
     m = m > g.nc2 ? m : g.nc2;
     g.tol0 = 2e-6f * (float)m;
@@ -1289,16 +1336,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __re
             const float* Aq = static_cast<const float*>(I->ws);
             switch (g.ndim) {
                 case 1: {
-                    const CellTable<1> tab = restage ? stage_tables<1>(smem, Aq, g) : table_view<1>(smem, g.nC);
+                    const CellTable<1> tab = restage ? stage_tables<1>(smem, Aq, g) : table_view<1>(smem, g.nC + g.nV);
                     fused_fwd_tile<1, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
                 } break;
                 case 2: {
-                    const CellTable<2> tab = restage ? stage_tables<2>(smem, Aq, g) : table_view<2>(smem, g.nC);
+                    const CellTable<2> tab = restage ? stage_tables<2>(smem, Aq, g) : table_view<2>(smem, g.nC + g.nV);
                     fused_fwd_tile<2, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
                 } break;
                 default:
                     if constexpr (MAXD == 3) {
-                        const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC);
+                        const CellTable<3> tab = restage ? stage_tables<3>(smem, Aq, g) : table_view<3>(smem, g.nC + g.nV);
                         fused_fwd_tile<3, SAVE>(L, g, s, im, tab, tile, im.data, I->out, I->saved);
                     }
                     break;
@@ -1315,7 +1362,7 @@ __device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, flo
     const int nCP = g.nC * P;
     float* acc = acc_base;
     for (int i = threadIdx.x; i < WAVES * nCP; i += BLOCK) acc[i] = 0.f;
-    const CellTable<D> tab = restage ? stage_tables<D>(smem, static_cast<const float*>(I->ws), g) : table_view<D>(smem, g.nC);
+    const CellTable<D> tab = restage ? stage_tables<D>(smem, static_cast<const float*>(I->ws), g) : table_view<D>(smem, g.nC + g.nV);
     if (!restage) __syncthreads();
     fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data, im.gdata, I->grad_out, I->saved,
                              acc + (threadIdx.x >> 6) * nCP, box, box_cap);
@@ -1690,6 +1737,7 @@ int check_tess(const cpab_tess_t* t) {
     for (int a = 0; a < t->ndim; ++a)
         if (t->nc[a] < 1) return CPAB_ERR_ARG;
     if (t->nC != cells_of(t->ndim, t->nc)) return CPAB_ERR_ARG;
+    if (t->nV < 0 || (t->nV > 0 && (t->ndim != 3 || !t->amap))) return CPAB_ERR_ARG;
     if (t->ndim > 1 && (!t->planes || !t->nbr)) return CPAB_ERR_ARG;
     return CPAB_OK;
 }
@@ -1704,6 +1752,8 @@ Geo make_geo(const cpab_tess_t* t) {
     g.pl = reinterpret_cast<const float4*>(t->planes);
     g.nb = t->nbr;
     g.unfinished = t->unfinished;
+    g.nV = t->nV;
+    g.amap = t->amap;
     int m = g.nc0 > g.nc1 ? g.nc0 : g.nc1;
     m = m > g.nc2 ? m : g.nc2;
     g.tol0 = 2e-6f * (float)(m > 1 ? m : 1);
@@ -1732,7 +1782,7 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
 // cells, a 32 x 16 tile at up to ~2x magnification)
 constexpr int BOX_FLOATS = 6144;
 
-size_t lds_tables(int ndim, int nC) {
+size_t lds_tables(int ndim, int nC) {  // nC: every cell of the facet table
     switch (ndim) {
         case 1: return table_floats<1>(nC) * 4;
         case 2: return table_floats<2>(nC) * 4;
@@ -1740,7 +1790,8 @@ size_t lds_tables(int ndim, int nC) {
     }
 }
 
-size_t lds_bwd(int ndim, int nC) { return lds_tables(ndim, nC) + (size_t)WAVES * nC * ndim * (ndim + 1) * 4; }
+// tables of all nCt cells, per-wave dL/dA accumulators of the nC simplices
+size_t lds_bwd(int ndim, int nC, int nCt) { return lds_tables(ndim, nCt) + (size_t)WAVES * nC * ndim * (ndim + 1) * 4; }
 
 // LDS one workgroup may allocate on gfx950 (160 KB per CU, all of it per block)
 constexpr size_t LDS_LIMIT = 160 * 1024;
@@ -1931,7 +1982,7 @@ int cpab_transform_fwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     if (n_theta == 0 || nP == 0) return CPAB_OK;  // arrays with no elements may be NULL
     if (!theta || !points || !out) return CPAB_ERR_ARG;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
-    const size_t lds = lds_tables(t->ndim, t->nC);
+    const size_t lds = lds_tables(t->ndim, t->nC + t->nV);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
@@ -1962,7 +2013,7 @@ int cpab_transform_cells(const cpab_tess_t* t, const float* theta, int n_theta, 
     if (n_theta == 0 || nP == 0) return CPAB_OK;
     if (!theta || !points || !nseg || (max_segs > 0 && !cells)) return CPAB_ERR_ARG;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
-    const size_t lds = lds_tables(t->ndim, t->nC);
+    const size_t lds = lds_tables(t->ndim, t->nC + t->nV);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     float* Aq = static_cast<float*>(workspace);
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
@@ -1995,7 +2046,7 @@ int cpab_transform_bwd(const cpab_tess_t* t, const float* theta, int n_theta, co
     // no points: grad_theta is written with zeros; points and grad_out may be NULL
     if (!theta || !grad_theta || (nP > 0 && (!points || !grad_out))) return CPAB_ERR_ARG;
     if (ws_bytes < cpab_workspace_bytes(t, n_theta, nP) || !workspace) return CPAB_ERR_WORKSPACE;
-    const size_t lds = lds_bwd(t->ndim, t->nC);
+    const size_t lds = lds_bwd(t->ndim, t->nC, t->nC + t->nV);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
     float* Aq = static_cast<float*>(workspace);
@@ -2132,7 +2183,7 @@ static int data_fwd(const cpab_tess_t* t, const float* theta, int n_theta, const
     L.nP = nP;
     L.ppt = pick_ppt(n_theta, nP);
     L.ty = tile_ty(t->ndim, s.g1, s.g2);
-    const size_t lds = lds_tables(t->ndim, t->nC);
+    const size_t lds = lds_tables(t->ndim, t->nC + t->nV);
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     if ((rc = launch_theta_to_A(t, theta, n_theta, Aq, stream))) return rc;
     s.lin = lin;
@@ -2184,10 +2235,10 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.ty = tile_ty(t->ndim, s.g1, s.g2);
     // 2D with saved state and dL/ddata: accumulate dL/ddata per tile in LDS
     const int box_cap = (t->ndim == 2 && L.ty > 0 && saved && grad_data &&
-                         lds_bwd(t->ndim, t->nC) + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT)
+                         lds_bwd(t->ndim, t->nC, t->nC + t->nV) + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT)
                             ? BOX_FLOATS
                             : 0;
-    const size_t lds = lds_bwd(t->ndim, t->nC) + (size_t)box_cap * 4;
+    const size_t lds = lds_bwd(t->ndim, t->nC, t->nC + t->nV) + (size_t)box_cap * 4;
     if (lds > LDS_LIMIT) return CPAB_ERR_LDS;
     const int Dt = t->nC * t->ndim * (t->ndim + 1);
     float* Aq = static_cast<float*>(workspace);
@@ -2344,7 +2395,7 @@ static int batch_check(const cpab_instance_t* h, int n_inst, int ppt, size_t& ld
         if (I.ws_bytes < cpab_batch_workspace_bytes(&I.tess, I.outsize, ppt)) return CPAB_ERR_WORKSPACE;
         const int nd = I.tess.ndim;
         const size_t Dt = (size_t)I.tess.nC * nd * (nd + 1);
-        const size_t tb = lds_tables(nd, I.tess.nC);
+        const size_t tb = lds_tables(nd, I.tess.nC + I.tess.nV);
         lds_tab = tb > lds_tab ? tb : lds_tab;
         lds_acc = (size_t)WAVES * Dt * 4 > lds_acc ? (size_t)WAVES * Dt * 4 : lds_acc;
         maxd = I.tess.d > maxd ? I.tess.d : maxd;

@@ -36,6 +36,10 @@ class DeviceTess:
         self.nbr = torch.as_tensor(np.ascontiguousarray(params.facet_nbr), dtype=torch.int32, device=device).contiguous()
         # points that used up the chunk budget in a forward (their outputs are NaN)
         self.unfinished = torch.zeros(1, dtype=torch.int32, device=device)
+        amap = np.asarray(getattr(params, "facet_amap", np.zeros(0)), dtype=np.int32)
+        if amap.size and (amap.min() < 0 or amap.max() >= self.nC or params.facet_nbr.shape[0] != self.nC + amap.size):
+            raise ValueError("facet_amap must name a simplex for each of the facet table's cells after the nC simplices")
+        self.amap = torch.as_tensor(amap if amap.size else np.zeros(1, np.int32), device=device).contiguous()
         t = N.CpabTess()
         t.ndim = self.ndim
         t.nc = N.int3(self.nc)
@@ -45,6 +49,8 @@ class DeviceTess:
         t.planes = self.planes.data_ptr()
         t.nbr = self.nbr.data_ptr()
         t.unfinished = self.unfinished.data_ptr()
+        t.nV = int(amap.size)
+        t.amap = self.amap.data_ptr()
         self.struct = t
         self.ref = ctypes.byref(t)
 

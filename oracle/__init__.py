@@ -193,14 +193,39 @@ def ref_backward(ndim, nc, nstep, points, A, Bs):
 # --------------------------------------------------------------------------
 # exact integrator
 # --------------------------------------------------------------------------
-def cf_forward(ndim, nc, A, planes, nbr, points, T=1.0, max_segs=0):
+def _virtual(ndim, nc, A, planes, points, amap):
+    """With virtual cells (amap non-empty: regions outside the domain, each
+    using the map of simplex amap[v]): A extended to every cell of the facet
+    table, and the start cell of each point outside [0,1]^ndim -- the region
+    whose largest facet value G = n.q + e is the smallest (the one that
+    contains it); -1 (the lookup) inside."""
+    if amap is None or len(amap) == 0:
+        return A, 0, None
+    amap = np.asarray(amap, dtype=np.int64)
+    nC = A.shape[1]
+    A_all = np.ascontiguousarray(np.concatenate([A, A[:, amap]], axis=1))
+    p = np.asarray(points, dtype=np.float64)
+    q = p * np.asarray(nc, dtype=np.float64).reshape((ndim, 1))
+    vp = np.asarray(planes, dtype=np.float64)[nC:]
+    used = np.any(vp[:, :, :ndim] != 0, axis=2)
+    G = np.einsum("vfa,...an->...vfn", vp[:, :, :ndim], q) + vp[:, :, 3][..., None]
+    G = np.where(used[:, :, None], G, -np.inf)
+    region = nC + np.argmin(G.max(axis=-2), axis=-2)
+    outside = np.any((p < 0.0) | (p > 1.0), axis=-2)
+    c0 = np.ascontiguousarray(np.where(outside, region, -1).astype(np.int32))
+    return A_all, A_all.shape[1], c0
+
+
+def cf_forward(ndim, nc, A, planes, nbr, points, T=1.0, max_segs=0, amap=None):
     """Exact flow of the CPA field for time T.  A [n_theta, nC, ndim, ndim+1].
     Returns out [n_theta, ndim, nP] (and cell sequences if max_segs > 0:
-    seq [n_theta, nP, max_segs], dur, nseg)."""
+    seq [n_theta, nP, max_segs], dur, nseg).  amap: the simplex of each
+    virtual cell of the facet table (3D without zero boundary)."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     planes = np.ascontiguousarray(planes, dtype=np.float64)
     nbr = np.ascontiguousarray(nbr, dtype=np.int32)
     points = np.ascontiguousarray(points, dtype=np.float64)
+    A, nC_all, c0 = _virtual(ndim, nc, A, planes, points, amap)
     n_theta = A.shape[0]
     nP = points.shape[-1]
     out = np.zeros((n_theta, ndim, nP))
@@ -212,28 +237,37 @@ def cf_forward(ndim, nc, A, planes, nbr, points, T=1.0, max_segs=0):
     else:
         args = (_i(0), None, None, None)
     lib().oracle_cf_forward(_i(ndim), _ptr(_nc(nc)), _ptr(A), _ptr(planes), _ptr(nbr), _ptr(points), _i(nP),
-                            _i(n_theta), _i(_bcast(points, n_theta)), _d(T), _ptr(out), *args)
+                            _i(n_theta), _i(_bcast(points, n_theta)), _d(T), _ptr(out), *args, _i(nC_all),
+                            _ptr(c0) if c0 is not None else None)
     if max_segs:
         return out, seq, dur, nseg
     return out
 
 
-def cf_backward(ndim, nc, A, planes, nbr, points, grad_out, T=1.0, saltation=True):
-    """Exact gradient of <grad_out, psi(points)> w.r.t. A and the points."""
+def cf_backward(ndim, nc, A, planes, nbr, points, grad_out, T=1.0, saltation=True, amap=None):
+    """Exact gradient of <grad_out, psi(points)> w.r.t. A and the points; a
+    virtual cell's share goes to the simplex whose map it uses."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     planes = np.ascontiguousarray(planes, dtype=np.float64)
     nbr = np.ascontiguousarray(nbr, dtype=np.int32)
     points = np.ascontiguousarray(points, dtype=np.float64)
     grad_out = np.ascontiguousarray(grad_out, dtype=np.float64)
+    nC = A.shape[1]
+    A_all, nC_all, c0 = _virtual(ndim, nc, A, planes, points, amap)
     n_theta = A.shape[0]
     nP = points.shape[-1]
-    gA = np.zeros_like(A)
+    gA = np.zeros_like(A_all)
     gp = np.zeros((n_theta, ndim, nP))
-    rc = lib().oracle_cf_backward(_i(ndim), _ptr(_nc(nc)), _ptr(A), _ptr(planes), _ptr(nbr), _ptr(points),
+    rc = lib().oracle_cf_backward(_i(ndim), _ptr(_nc(nc)), _ptr(A_all), _ptr(planes), _ptr(nbr), _ptr(points),
                                   _i(nP), _i(n_theta), _i(_bcast(points, n_theta)), _d(T), _ptr(grad_out),
-                                  _ptr(gA), _ptr(gp), _i(int(saltation)))
+                                  _ptr(gA), _ptr(gp), _i(int(saltation)), _i(nC_all),
+                                  _ptr(c0) if c0 is not None else None)
     if rc != 0:
         raise RuntimeError("oracle_cf_backward failed: %d" % rc)
+    if nC_all:
+        g = gA[:, :nC].copy()
+        np.add.at(g, (slice(None), np.asarray(amap, dtype=np.int64)), gA[:, nC:])
+        gA = g
     return gA, gp
 
 

@@ -508,11 +508,12 @@ static double first_exit(const double* c, double h, double vscale, int allow_sta
 /* integrate one point for time T; returns number of chunks written (if chunks != NULL) */
 static int integrate_point(const geom_t* g, const double* xin, double T, double* xout,
                            chunk_t* chunks, int max_chunks, int* seg_cells, double* seg_dur, int max_segs,
-                           int* n_segs_out) {
+                           int* n_segs_out, int c_start) {
     int n = g->ndim;
     double x[3] = {0.0, 0.0, 0.0};
     for (int a = 0; a < n; ++a) x[a] = xin[a];
-    int c = np_find(n, g->nc, x);
+    /* c_start >= 0: the caller's start cell (a region outside the domain) */
+    int c = c_start >= 0 ? c_start : np_find(n, g->nc, x);
     int nch = 0, nseg = 0;
     double seg_t = 0;
     int came_from = -1, zero_hops = 0;
@@ -596,15 +597,18 @@ static void setup_geom(geom_t* g, int ndim, const int* nc, const double* A, cons
     g->nbr = nbr;
 }
 
-/* Exact forward. A: [n_theta, nC, ndim, ndim+1]; points [ndim, nP] or [n_theta, ndim, nP].
- * out [n_theta, ndim, nP]. Optional cell sequences: seq [n_theta, nP, max_segs] (-1 padded),
- * nseg [n_theta, nP], durations dur [n_theta, nP, max_segs]. */
+/* Exact forward. A: [n_theta, nC_all, ndim, ndim+1] (nC_all = 0: the nC simplices; more:
+ * virtual cells after them, each with its own copy of the map it uses); points [ndim, nP] or
+ * [n_theta, ndim, nP]; c0 (nullable, laid out like the points without the ndim axis): start
+ * cells, -1 for the lookup.  out [n_theta, ndim, nP]. Optional cell sequences: seq [n_theta,
+ * nP, max_segs] (-1 padded), nseg [n_theta, nP], durations dur [n_theta, nP, max_segs]. */
 int oracle_cf_forward(int ndim, const int* nc, const double* A, const double* planes, const int* nbr,
                       const double* points, int nP, int n_theta, int broadcast, double T,
-                      double* out, int max_segs, int* seq, double* dur, int* nseg) {
+                      double* out, int max_segs, int* seq, double* dur, int* nseg, int nC_all, const int* c0) {
     geom_t g;
+    if (nC_all <= 0) nC_all = n_cells(ndim, nc);
     for (int t = 0; t < n_theta; ++t) {
-        setup_geom(&g, ndim, nc, A + (size_t)t * n_cells(ndim, nc) * ndim * (ndim + 1), planes, nbr);
+        setup_geom(&g, ndim, nc, A + (size_t)t * nC_all * ndim * (ndim + 1), planes, nbr);
         for (int i = 0; i < nP; ++i) {
             double x[3], y[3];
             for (int a = 0; a < ndim; ++a) x[a] = points[(size_t)broadcast * t * nP * ndim + a * nP + i];
@@ -612,7 +616,8 @@ int oracle_cf_forward(int ndim, const int* nc, const double* A, const double* pl
             int* sq = seq ? seq + ((size_t)t * nP + i) * max_segs : NULL;
             double* du = dur ? dur + ((size_t)t * nP + i) * max_segs : NULL;
             if (sq) for (int s = 0; s < max_segs; ++s) { sq[s] = -1; du[s] = 0; }
-            integrate_point(&g, x, T, y, NULL, 0, sq, du, sq ? max_segs : 0, &ns);
+            const int cs = c0 ? c0[(size_t)broadcast * t * nP + i] : -1;
+            integrate_point(&g, x, T, y, NULL, 0, sq, du, sq ? max_segs : 0, &ns, cs);
             if (nseg) nseg[(size_t)t * nP + i] = ns;
             for (int a = 0; a < ndim; ++a) out[(size_t)t * ndim * nP + a * nP + i] = y[a];
         }
@@ -671,13 +676,15 @@ static void chunk_adjoint(int n, double M[3][3], const double* x0, const double*
     for (int a = 0; a < n; ++a) lam[a] = nl[a];
 }
 
-/* Exact gradient. grad_out [n_theta, ndim, nP]; grad_A [n_theta, nC, ndim, ndim+1] (overwritten);
- * grad_pts [n_theta, ndim, nP] (nullable). saltation: apply the facet-crossing terms. */
+/* Exact gradient. grad_out [n_theta, ndim, nP]; grad_A [n_theta, nC_all, ndim, ndim+1]
+ * (overwritten; nC_all, c0 as in oracle_cf_forward); grad_pts [n_theta, ndim, nP] (nullable).
+ * saltation: apply the facet-crossing terms. */
 int oracle_cf_backward(int ndim, const int* nc, const double* A, const double* planes, const int* nbr,
                        const double* points, int nP, int n_theta, int broadcast, double T,
-                       const double* grad_out, double* grad_A, double* grad_pts, int saltation) {
+                       const double* grad_out, double* grad_A, double* grad_pts, int saltation, int nC_all,
+                       const int* c0) {
     geom_t g;
-    int nC = n_cells(ndim, nc);
+    int nC = nC_all > 0 ? nC_all : n_cells(ndim, nc);
     int P = ndim * (ndim + 1);
     chunk_t* ch = (chunk_t*)malloc(sizeof(chunk_t) * OR_MAXCH);
     if (!ch) return -1;
@@ -689,7 +696,8 @@ int oracle_cf_backward(int ndim, const int* nc, const double* A, const double* p
         for (int i = 0; i < nP; ++i) {
             double x[3], y[3];
             for (int a = 0; a < ndim; ++a) x[a] = points[(size_t)broadcast * t * nP * ndim + a * nP + i];
-            int nch = integrate_point(&g, x, T, y, ch, OR_MAXCH, NULL, NULL, 0, NULL);
+            const int cs = c0 ? c0[(size_t)broadcast * t * nP + i] : -1;
+            int nch = integrate_point(&g, x, T, y, ch, OR_MAXCH, NULL, NULL, 0, NULL, cs);
             if (nch > OR_MAXCH) { free(ch); return -2; }
             double lam[3];
             for (int a = 0; a < ndim; ++a) lam[a] = grad_out[(size_t)t * ndim * nP + a * nP + i];

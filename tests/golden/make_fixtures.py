@@ -10,7 +10,9 @@ archives' timestamps even when the numbers are the same).
 
 Fixtures (all .npz, no pickles):
   basis_<name>.npz   L-derived basis B [D, d] (float64), facet planes and
-                     neighbours of the tessellations the tests use.  theta
+                     neighbours of the tessellations the tests use (and
+                     amap, the simplex of each region outside the domain,
+                     for 3D without zero boundary).  theta
                      only has a meaning together with B, so every test that
                      compares against stored numbers uses these B.
   ref_step_<name>.npz  inputs and outputs of the reference's compiled
@@ -40,6 +42,7 @@ TESS = {
     "2d_2x2_nzb": ([2, 2], False),
     "3d_2x2x2": ([2, 2, 2], True),
     "3d_3x3x3": ([3, 3, 3], True),
+    "3d_2x2x2_nzb": ([2, 2, 2], False),
 }
 
 STEP_CASES = {  # name: (tess, n_theta, nP, theta scale)
@@ -59,7 +62,7 @@ def main():
         t = make_tessellation(nc, zero_boundary=zb)
         planes, nbr = t.facet_table()
         np.savez_compressed(os.path.join(HERE, "basis_%s.npz" % name), nc=np.array(nc), zero_boundary=zb,
-                            B=t.B, planes=planes, nbr=nbr)
+                            B=t.B, planes=planes, nbr=nbr, amap=t.amap)
     rng = np.random.default_rng(20240601)
     for name, (tname, n_theta, nP, scale) in STEP_CASES.items():
         if os.path.exists(os.path.join(HERE, "ref_step_%s.npz" % name)) and not force:

@@ -18,6 +18,7 @@ PROBLEMS = [  # tess, zero_boundary, channels, data size, output size
     ([2, 2, 2], True, 1, [9, 8, 7], [9, 8, 7]),
     ([2, 2, 2], True, 2, [10, 9, 8], [11, 9, 8]),  # 3D output tiles
     ([8, 8], True, 3, [48, 40], [48, 40]),  # tables leave no LDS for the dL/ddata box
+    ([2, 2, 2], False, 1, [8, 7, 9], [8, 9, 7]),  # regions outside the domain
 ]
 
 

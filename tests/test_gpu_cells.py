@@ -19,12 +19,13 @@ EPS = 1e-6
 def _oracle_seqs(b, A, pts):
     nd = pts.shape[0]
     _, seq, dur, nseg = O.cf_forward(nd, [int(v) for v in b["nc"]], A, b["planes"], b["nbr"], pts,
-                                     max_segs=MAX_SEGS)
+                                     max_segs=MAX_SEGS, amap=b.get("amap"))
     return seq, nseg
 
 
 @pytest.mark.parametrize("name,scale", [("2d_4x4", 0.5), ("2d_3x3", 1.0), ("2d_2x2_nzb", 0.7), ("1d_10", 1.0),
-                                        ("1d_50", 0.5), ("3d_2x2x2", 0.5), ("3d_3x3x3", 0.3)])
+                                        ("1d_50", 0.5), ("3d_2x2x2", 0.5), ("3d_3x3x3", 0.3),
+                                        ("3d_2x2x2_nzb", 0.6)])
 def test_cell_sequences_match_the_oracle(cuda, name, scale):
     T, b = make_cpab(name)
     nd = len(b["nc"])

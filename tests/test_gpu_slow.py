@@ -34,7 +34,8 @@ def _stable(S, pts, tr, nc):
     return ok
 
 
-@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("2d_3x3", 0.5), ("2d_2x2_nzb", 0.5), ("3d_2x2x2", 0.3)])
+@pytest.mark.parametrize("name,scale", [("1d_10", 0.8), ("2d_3x3", 0.5), ("2d_2x2_nzb", 0.5), ("3d_2x2x2", 0.3),
+                                        ("3d_2x2x2_nzb", 0.3)])
 def test_use_slow_matches_its_fp64_restatement(cuda, name, scale):
     from oracle import slow_path as S
     T, b = make_cpab(name)
