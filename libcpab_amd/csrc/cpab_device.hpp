@@ -1145,6 +1145,36 @@ __device__ __forceinline__ int wave_max(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+// One DPP step on every row: adds the value of the lane CTRL selects (0
+// where there is none), a single add with a DPP operand.
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// wave_total of P values at once, read from lane 63: every step is taken for
+// all P before the next, so the steps of one value do not wait on each
+// other.  The two row broadcasts run on all rows; the rows wave_total masks
+// out (0 and 2, then 0 and 1) end with other sums, but lane 63 -- which
+// reads only the rows those steps keep -- gets wave_total's sum bit for bit.
+template <int P>
+__device__ __forceinline__ void wave_totals(float (&v)[P]) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = dpp_add<0x111>(v[p]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = dpp_add<0x112>(v[p]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = dpp_add<0x114>(v[p]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = dpp_add<0x118>(v[p]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = dpp_add<0x142>(v[p]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = dpp_add<0x143>(v[p]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) v[p] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[p]), 63));
+}
+
 // Adds g (P values) of every lane with cell >= 0 into acc[cell*P + p] of this
 // wave's LDS region; the lanes sharing a cell are summed across the wave in a
 // fixed order, one lane writes.  Must be reached by all lanes of the wave.
@@ -1159,7 +1189,8 @@ __device__ __forceinline__ void wave_accumulate(float* acc, int cell, const floa
         pending &= ~__ballot(mine);
         float tot[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) tot[p] = wave_total(mine ? g[p] : 0.f);
+        for (int p = 0; p < P; ++p) tot[p] = mine ? g[p] : 0.f;
+        wave_totals<P>(tot);
         if (lane == leader) {
 #pragma unroll
             for (int p = 0; p < P; ++p) acc[c0 * P + p] += tot[p];

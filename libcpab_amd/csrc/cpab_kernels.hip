@@ -353,37 +353,6 @@ __device__ __forceinline__ void axis_corners(float x, int size, int& i0, int& i1
     fd = xs - (float)a0;
 }
 
-// value of channel c at point y (x-space)
-template <int D>
-__device__ __forceinline__ float interp_value(const float* __restrict__ dp, const Img& im, const int (&i0)[3],
-                                              const int (&i1)[3], const float (&fd)[3]) {
-    if constexpr (D == 1) {
-        const float c0 = dp[i0[0]], c1 = dp[i1[0]];
-        return c0 * (1.f - fd[0]) + c1 * fd[0];
-    } else if constexpr (D == 2) {
-        const int H = im.s1;
-        const float c00 = dp[i0[0] * H + i0[1]], c01 = dp[i0[0] * H + i1[1]];
-        const float c10 = dp[i1[0] * H + i0[1]], c11 = dp[i1[0] * H + i1[1]];
-        const float c0 = c00 * (1.f - fd[0]) + c10 * fd[0];
-        const float c1 = c01 * (1.f - fd[0]) + c11 * fd[0];
-        return c0 * (1.f - fd[1]) + c1 * fd[1];
-    } else {
-        const int H = im.s1, Z = im.s2;
-        auto at = [&](int x, int y, int z) { return dp[(x * H + y) * Z + z]; };
-        const float c000 = at(i0[0], i0[1], i0[2]), c001 = at(i0[0], i0[1], i1[2]);
-        const float c010 = at(i0[0], i1[1], i0[2]), c011 = at(i0[0], i1[1], i1[2]);
-        const float c100 = at(i1[0], i0[1], i0[2]), c101 = at(i1[0], i0[1], i1[2]);
-        const float c110 = at(i1[0], i1[1], i0[2]), c111 = at(i1[0], i1[1], i1[2]);
-        const float c00 = c000 * (1.f - fd[0]) + c100 * fd[0];
-        const float c01 = c001 * (1.f - fd[0]) + c101 * fd[0];
-        const float c10 = c010 * (1.f - fd[0]) + c110 * fd[0];
-        const float c11 = c011 * (1.f - fd[0]) + c111 * fd[0];
-        const float c0 = c00 * (1.f - fd[1]) + c10 * fd[1];
-        const float c1 = c01 * (1.f - fd[1]) + c11 * fd[1];
-        return c0 * (1.f - fd[2]) + c1 * fd[2];
-    }
-}
-
 template <int D>
 __device__ __forceinline__ void corners(const float (&y)[D], const Img& im, int (&i0)[3], int (&i1)[3], float (&fd)[3]) {
     const int sz[3] = {im.s0, im.s1, im.s2};
@@ -397,38 +366,124 @@ __device__ __forceinline__ void corners(const float (&y)[D], const Img& im, int 
     for (int a = 0; a < D; ++a) axis_corners(y[a], sz[a], i0[a], i1[a], fd[a]);
 }
 
-// backward of interp_value for channel data dp with upstream g: scatters into
-// gp (nullable) and returns d value / d fd[a] accumulated into dfd
+// Offsets of the 2^D interpolation corners in one channel's data, corner
+// j = sum_a bit_a 2^(D-1-a) taking i1 on axis a where bit_a is set (2D:
+// c00, c01, c10, c11 with the second index the contiguous axis).
 template <int D>
-__device__ __forceinline__ void interp_backward(const float* __restrict__ dp, float* gp, const Img& im, const int (&i0)[3],
-                                                const int (&i1)[3], const float (&fd)[3], float g, float (&dfd)[3]) {
+__device__ __forceinline__ void corner_offsets(const Img& im, const int (&i0)[3], const int (&i1)[3],
+                                               int (&off)[1 << D]) {
     if constexpr (D == 1) {
-        const float c0 = dp[i0[0]], c1 = dp[i1[0]];
-        if (gp) {
-            atomicAdd(gp + i0[0], g * (1.f - fd[0]));
-            atomicAdd(gp + i1[0], g * fd[0]);
-        }
-        dfd[0] = fmaf(g, c1 - c0, dfd[0]);
+        off[0] = i0[0];
+        off[1] = i1[0];
     } else if constexpr (D == 2) {
         const int H = im.s1;
-        const float c00 = dp[i0[0] * H + i0[1]], c01 = dp[i0[0] * H + i1[1]];
-        const float c10 = dp[i1[0] * H + i0[1]], c11 = dp[i1[0] * H + i1[1]];
+        off[0] = i0[0] * H + i0[1];
+        off[1] = i0[0] * H + i1[1];
+        off[2] = i1[0] * H + i0[1];
+        off[3] = i1[0] * H + i1[1];
+    } else {
+        const int H = im.s1, Z = im.s2;
+        const int ii[3][2] = {{i0[0], i1[0]}, {i0[1], i1[1]}, {i0[2], i1[2]}};
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+            for (int by = 0; by < 2; ++by)
+#pragma unroll
+                for (int bz = 0; bz < 2; ++bz) off[bx * 4 + by * 2 + bz] = (ii[0][bx] * H + ii[1][by]) * Z + ii[2][bz];
+    }
+}
+
+// the interpolated value from the corner values v (corner_offsets order)
+template <int D>
+__device__ __forceinline__ float blend(const float (&v)[1 << D], const float (&fd)[3]) {
+    if constexpr (D == 1) {
+        return v[0] * (1.f - fd[0]) + v[1] * fd[0];
+    } else if constexpr (D == 2) {
+        const float c0 = v[0] * (1.f - fd[0]) + v[2] * fd[0];
+        const float c1 = v[1] * (1.f - fd[0]) + v[3] * fd[0];
+        return c0 * (1.f - fd[1]) + c1 * fd[1];
+    } else {
+        const float c00 = v[0] * (1.f - fd[0]) + v[4] * fd[0];
+        const float c01 = v[1] * (1.f - fd[0]) + v[5] * fd[0];
+        const float c10 = v[2] * (1.f - fd[0]) + v[6] * fd[0];
+        const float c11 = v[3] * (1.f - fd[0]) + v[7] * fd[0];
+        const float c0 = c00 * (1.f - fd[1]) + c10 * fd[1];
+        const float c1 = c01 * (1.f - fd[1]) + c11 * fd[1];
+        return c0 * (1.f - fd[2]) + c1 * fd[2];
+    }
+}
+
+// value of channel data dp at the corners (i0, i1, fd)
+template <int D>
+__device__ __forceinline__ float interp_value(const float* __restrict__ dp, const Img& im, const int (&i0)[3],
+                                              const int (&i1)[3], const float (&fd)[3]) {
+    int off[1 << D];
+    corner_offsets<D>(im, i0, i1, off);
+    float v[1 << D];
+#pragma unroll
+    for (int j = 0; j < (1 << D); ++j) v[j] = dp[off[j]];
+    return blend<D>(v, fd);
+}
+
+// Channels whose loads one group issues together.  The forward keeps one (a
+// group of four, measured on the 2D image workload, gained nothing and cost
+// it registers); the 2D backward takes two (the others keep their own loop).
+template <int D>
+constexpr int chan_group() { return 1; }
+template <int D>
+constexpr int chan_group_bwd() { return 2; }
+
+// out[c * ostride] = channel c (data + c * dstride) at the corners, for every
+// channel.  The loads of a group of channels are all issued before the first
+// of them is used, so their latencies overlap instead of adding up.
+template <int D>
+__device__ __forceinline__ void interp_channels(const float* __restrict__ data, size_t dstride, const Img& im,
+                                                const int (&i0)[3], const int (&i1)[3], const float (&fd)[3],
+                                                float* __restrict__ out, size_t ostride) {
+    constexpr int NK = 1 << D, CG = chan_group<D>();
+    int off[NK];
+    corner_offsets<D>(im, i0, i1, off);
+    for (int c0 = 0; c0 < im.C; c0 += CG) {
+        float v[CG][NK];
+#pragma unroll
+        for (int k = 0; k < CG; ++k)
+            if (c0 + k < im.C) {
+                const float* dp = data + (size_t)(c0 + k) * dstride;
+#pragma unroll
+                for (int j = 0; j < NK; ++j) v[k][j] = dp[off[j]];
+            }
+#pragma unroll
+        for (int k = 0; k < CG; ++k)
+            if (c0 + k < im.C) out[(size_t)(c0 + k) * ostride] = blend<D>(v[k], fd);
+    }
+}
+
+// backward of blend for upstream g at corner values v: adds d value / d fd[a]
+// into dfd and, with gp (nullable), scatters g times each corner's weight
+template <int D>
+__device__ __forceinline__ void blend_backward(const float (&v)[1 << D], const int (&off)[1 << D], float* gp,
+                                               const float (&fd)[3], float g, float (&dfd)[3]) {
+    if constexpr (D == 1) {
+        if (gp) {
+            atomicAdd(gp + off[0], g * (1.f - fd[0]));
+            atomicAdd(gp + off[1], g * fd[0]);
+        }
+        dfd[0] = fmaf(g, v[1] - v[0], dfd[0]);
+    } else if constexpr (D == 2) {
+        const float c00 = v[0], c01 = v[1], c10 = v[2], c11 = v[3];
         const float wx0 = 1.f - fd[0], wx1 = fd[0], wy0 = 1.f - fd[1], wy1 = fd[1];
         if (gp) {
-            atomicAdd(gp + i0[0] * H + i0[1], g * wx0 * wy0);
-            atomicAdd(gp + i1[0] * H + i0[1], g * wx1 * wy0);
-            atomicAdd(gp + i0[0] * H + i1[1], g * wx0 * wy1);
-            atomicAdd(gp + i1[0] * H + i1[1], g * wx1 * wy1);
+            atomicAdd(gp + off[0], g * wx0 * wy0);
+            atomicAdd(gp + off[2], g * wx1 * wy0);
+            atomicAdd(gp + off[1], g * wx0 * wy1);
+            atomicAdd(gp + off[3], g * wx1 * wy1);
         }
         const float c0 = c00 * wx0 + c10 * wx1;
         const float c1 = c01 * wx0 + c11 * wx1;
         dfd[0] = fmaf(g, (c10 - c00) * wy0 + (c11 - c01) * wy1, dfd[0]);
         dfd[1] = fmaf(g, c1 - c0, dfd[1]);
     } else {
-        const int H = im.s1, Z = im.s2;
-        auto idx = [&](int x, int y, int z) { return (x * H + y) * Z + z; };
         const float w[3][2] = {{1.f - fd[0], fd[0]}, {1.f - fd[1], fd[1]}, {1.f - fd[2], fd[2]}};
-        const int ii[3][2] = {{i0[0], i1[0]}, {i0[1], i1[1]}, {i0[2], i1[2]}};
         float cv[2][2][2];
 #pragma unroll
         for (int bx = 0; bx < 2; ++bx)
@@ -436,9 +491,9 @@ __device__ __forceinline__ void interp_backward(const float* __restrict__ dp, fl
             for (int by = 0; by < 2; ++by)
 #pragma unroll
                 for (int bz = 0; bz < 2; ++bz) {
-                    const int o = idx(ii[0][bx], ii[1][by], ii[2][bz]);
-                    cv[bx][by][bz] = dp[o];
-                    if (gp) atomicAdd(gp + o, g * w[0][bx] * w[1][by] * w[2][bz]);
+                    const int j = bx * 4 + by * 2 + bz;
+                    cv[bx][by][bz] = v[j];
+                    if (gp) atomicAdd(gp + off[j], g * w[0][bx] * w[1][by] * w[2][bz]);
                 }
         // derivative of the trilinear blend w.r.t. fd[a]
         float dx = 0.f, dy = 0.f, dz = 0.f;
@@ -457,6 +512,54 @@ __device__ __forceinline__ void interp_backward(const float* __restrict__ dp, fl
         dfd[0] = fmaf(g, dx, dfd[0]);
         dfd[1] = fmaf(g, dy, dfd[1]);
         dfd[2] = fmaf(g, dz, dfd[2]);
+    }
+}
+
+// backward of interp_value for channel data dp with upstream g: scatters into
+// gp (nullable) and returns d value / d fd[a] accumulated into dfd
+template <int D>
+__device__ __forceinline__ void interp_backward(const float* __restrict__ dp, float* gp, const Img& im, const int (&i0)[3],
+                                                const int (&i1)[3], const float (&fd)[3], float g, float (&dfd)[3]) {
+    int off[1 << D];
+    corner_offsets<D>(im, i0, i1, off);
+    float v[1 << D];
+#pragma unroll
+    for (int j = 0; j < (1 << D); ++j) v[j] = dp[off[j]];
+    blend_backward<D>(v, off, gp, fd, g, dfd);
+}
+
+// interp_backward over every channel c, upstream gout[c * gstride] (the first
+// four also returned in gk): the loads of a group of channels' upstream values
+// and corners are issued before any of them is used; dfd accumulates in
+// channel order.  gdata (nullable) receives the scatter.
+template <int D>
+__device__ __forceinline__ void interp_backward_channels(const float* __restrict__ data, float* gdata, size_t dstride,
+                                                         const Img& im, const int (&i0)[3], const int (&i1)[3],
+                                                         const float (&fd)[3], const float* __restrict__ gout,
+                                                         size_t gstride, float (&gk)[4], float (&dfd)[3]) {
+    constexpr int NK = 1 << D, CG = chan_group_bwd<D>();
+    int off[NK];
+    corner_offsets<D>(im, i0, i1, off);
+    for (int c0 = 0; c0 < im.C; c0 += CG) {
+        float v[CG][NK], gv[CG];
+#pragma unroll
+        for (int k = 0; k < CG; ++k)
+            if (c0 + k < im.C) {
+                const float* dp = data + (size_t)(c0 + k) * dstride;
+                gv[k] = gout[(size_t)(c0 + k) * gstride];
+#pragma unroll
+                for (int j = 0; j < NK; ++j) v[k][j] = dp[off[j]];
+            }
+#pragma unroll
+        for (int k = 0; k < CG; ++k)
+            if (c0 + k < im.C) {
+                blend_backward<D>(v[k], off, gdata ? gdata + (size_t)(c0 + k) * dstride : nullptr, fd, gv[k], dfd);
+                if (c0 + k < 4) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (j == c0 + k) gk[j] = gv[k];
+                }
+            }
     }
 }
 
@@ -636,8 +739,7 @@ __device__ __forceinline__ void fused_fwd_tile(const Launch& L, const Geo& g, co
         int i0[3], i1[3];
         float fd[3];
         corners<D>(y, im, i0, i1, fd);
-        for (int c = 0; c < im.C; ++c)
-            out[(size_t)c * L.nP + t] = interp_value<D>(data + (size_t)c * dstride, im, i0, i1, fd);
+        interp_channels<D>(data, dstride, im, i0, i1, fd, out + t, L.nP);
         wave_bad += __popcll(__ballot(y[0] != y[0]));
     }
     if (wave_bad && g.unfinished && __lane_id() == 0) atomicAdd(g.unfinished, wave_bad);
@@ -879,12 +981,16 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             corners<D>(y, im, i0, i1, fd);
             const int slot = r * BLOCK + threadIdx.x;
             float gk[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int c = 0; c < im.C; ++c) {
-                const size_t base = (size_t)c * dstride;
-                const float gv = gout[(size_t)c * L.nP + t];
-                interp_backward<D>(data + base, (gdata && !use_box) ? gdata + base : nullptr, im, i0, i1, fd, gv,
-                                   dfd);
-                if (c < 4) gk[c] = gv;
+            if constexpr (D == 2) {
+                interp_backward_channels<D>(data, (gdata && !use_box) ? gdata : nullptr, dstride, im, i0, i1, fd,
+                                            gout + t, L.nP, gk, dfd);
+            } else {
+                for (int c = 0; c < im.C; ++c) {
+                    const size_t base = (size_t)c * dstride;
+                    const float gv = gout[(size_t)c * L.nP + t];
+                    interp_backward<D>(data + base, gdata ? gdata + base : nullptr, im, i0, i1, fd, gv, dfd);
+                    if (c < 4) gk[c] = gv;
+                }
             }
             if constexpr (D == 2) {
                 if (use_box) {
