@@ -517,13 +517,14 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
     // chunk, until the chunk budget ran out).
     const bool still = vn <= tol1;
     // Bounds on the chunk (|n| = 1, |M| <= |M|_F, |M|_F h <= RHO, |cf_k| <=
-    // |M|^(k-1) |v0| / k!):  |G''(s)| <= K = |M|_F |v0| e^RHO and
+    // |M|^(k-1) |v0| / k!):  the terms of G from s^3 on sum to at most
+    // K3 s^3, K3 = |M|_F^2 |v0| e^RHO / 6, their derivative to 3 K3 s^2, and
     // |G''(s) - 2 cf2 - 6 cf3 s| <= K4 = |M|_F^3 |v0| h^2 e^RHO / 2.
     const float nrm = fast_rcp(tab.inv_n[c]);  // |M|_F (the bounds below carry margin for its last bit)
     const float mh = h * nrm;
     const float K = GROW * vn * nrm;
     const float K4 = 0.5f * K * mh * mh;
-    const float k2h2 = 0.5f * K * h * h;
+    const float K3 = K * nrm * (1.0f / 6.0f);
 
     // phase 1: classify every facet.  A facet whose first crossing lies in a
     // bracket with a unique root becomes a candidate; starts on a facet, and
@@ -547,13 +548,17 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
             g0 = fmaf(n[a], st.q[a], g0);
             g1 = fmaf(n[a], v0[a], g1);
         }
-        const bool on = g0 >= -tol0;
-        // G(s) <= g0 + g1 s + K s^2/2 on [0, h]: no crossing when that stays below 0
-        if (!on && fmaf(g1, h, g0) + k2h2 < 0.f) continue;
-        // start on the facet moving inwards with G' < 0 on all of [0, h]: no return
-        if (on && g1 < -tol1 && fmaf(K, h, g1) < 0.f) continue;
 #pragma unroll
         for (int a = 0; a < D; ++a) g2 = fmaf(n[a], ch.W[1][a], g2);
+        const bool on = g0 >= -tol0;
+        // G(s) <= g0 + g1 s + max(cf2, 0) s^2 + K3 s^3 on [0, h], a convex bound
+        // (its s^2 and s^3 coefficients are >= 0), so no crossing when it is
+        // below 0 at h
+        const float c2p = fmaxf(g2, 0.f);
+        if (!on && fmaf(h, fmaf(h, fmaf(h, K3, c2p), g1), g0) < 0.f) continue;
+        // start on the facet moving inwards with G' <= g1 + 2 max(cf2, 0) s +
+        // 3 K3 s^2 < 0 on all of [0, h]: no return
+        if (on && g1 < -tol1 && fmaf(h, fmaf(h, 3.f * K3, 2.f * c2p), g1) < 0.f) continue;
         if (on) {
             const bool allow_start = !(st.zero_hops > 0 && nbf == st.came_from) && st.zero_hops < MAX_ZERO_HOPS;
             if (allow_start && (g1 > tol1 || (g1 >= 0.f && g2 > 0.f))) {

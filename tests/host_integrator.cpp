@@ -152,7 +152,8 @@ static int replay_mismatches(const int* nc, int nC, const float* Aq, const float
         logged += nrec;
         // the forward's memory sink records the same log
         {
-            float hs[NREC] = {-1.f, -1.f};
+            float hs[NREC];
+            for (int k = 0; k < NREC; ++k) hs[k] = -1.f;
             LogSink sk;
             sk.init(hs, 1, 0);
             float qs[D], Js[D][D];
