@@ -87,6 +87,16 @@ CPAB_HD __forceinline__ float fast_rcp(float x) {
     return 1.0f / x;
 #endif
 }
+// sqrt(x) to ~1 ulp for the chunk bounds and thresholds (their constants
+// carry the margin; the correctly rounded square root costs a dozen more
+// instructions per chunk)
+CPAB_HD __forceinline__ float fast_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
 template <int D>
 CPAB_HD __forceinline__ void load_affine(const CellTable<D>& t, int c, float (&M)[D][D], float (&b)[D]) {
     const float* a = t.A + c * (D * (D + 1));
@@ -493,7 +503,7 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
     float vn2 = 0.f;
 #pragma unroll
     for (int a = 0; a < D; ++a) vn2 = fmaf(v0[a], v0[a], vn2);
-    const float vn = sqrtf(vn2);
+    const float vn = fast_sqrt(vn2);
     // Tangential band for an outward normal speed G'(0): the neighbouring
     // cells' FP32 fields agree on a shared facet only up to the rounding of
     // M q + b, whose terms can be much larger than |v0| where they cancel, so
@@ -531,13 +541,10 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
     // facets whose curvature sign is not certified, take the sampling path.
     int start_f = -1;
     unsigned cand = 0u, slow = 0u;
+    // bracket of each candidate facet (set only where cand has its bit)
     float clo[NF], chi[NF], cglo[NF], cghi[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-        clo[f] = 0.f;
-        chi[f] = h;
-        cglo[f] = 0.f;
-        cghi[f] = 0.f;
         const int nbf = tab.nb[c * NF + f];
         if (nbf < 0 || still) continue;
         const float4 P = tab.pl[c * NF + f];
@@ -593,6 +600,8 @@ CPAB_HD __forceinline__ void advance(PointState<D>& st, const CellTable<D>& tab,
         horner2(cf, h, gh, dh);
         if (gh > 0.f) {
             cand |= 1u << f;
+            clo[f] = 0.f;
+            chi[f] = h;
             cglo[f] = g0;
             cghi[f] = gh;
         } else if (g2 < 0.f && g1 > 0.f && dh < 0.f) {
@@ -867,8 +876,8 @@ CPAB_HD __forceinline__ void saltation_fwd(const CellTable<D>& tab, const Chunk<
         nv = fmaf(n[a], vp[a], nv);
         vv = fmaf(vp[a], vp[a], vv);
     }
-    if (fabsf(nv) <= 1e-6f * sqrtf(vv)) return;
-    const float inv = 1.0f / nv;
+    if (fabsf(nv) <= 1e-6f * fast_sqrt(vv)) return;
+    const float inv = fast_rcp(nv);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         float nj = 0.f;
@@ -895,8 +904,8 @@ CPAB_HD __forceinline__ void saltation_adj(const CellTable<D>& tab, const Chunk<
         vv = fmaf(vn[a], vn[a], vv);
         um = fmaf(vn[a] - vp[a], mu[a], um);
     }
-    if (fabsf(nv) <= 1e-6f * sqrtf(vv)) return;
-    const float f = um / nv;
+    if (fabsf(nv) <= 1e-6f * fast_sqrt(vv)) return;
+    const float f = um * fast_rcp(nv);
 #pragma unroll
     for (int a = 0; a < D; ++a) mu[a] = fmaf(-n[a], f, mu[a]);
 }
