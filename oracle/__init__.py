@@ -12,7 +12,11 @@ Pieces (C functions live in oracle/cpab_oracle.c, built by oracle/Makefile):
                             libcpab/core/cpab_ops.cpp:224-372
   trels                     Trel = expm(A~ / nstep) per cell,
                             libcpab/numpy/transformer.py:40-47
-  cf_forward / cf_backward  exact FP64 integration and its gradient
+  cf_forward / cf_backward  exact FP64 integration and its gradient; with
+                            amap (3D without zero boundary) also through the
+                            regions outside the domain, each with the map of
+                            the tetrahedron libcpab/numpy/findcellidx.py:84-109
+                            gives its pushed points
   interpolate / interpolate_backward
                             libcpab/pytorch/interpolation.py:18-171 in FP64
   ref_forward / ref_backward
