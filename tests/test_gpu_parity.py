@@ -170,6 +170,33 @@ def test_transform_data_fused_matches_unfused(cuda, name, n_theta, scale, size, 
     assert np.abs(out1.detach().cpu().numpy() - ref).max() <= POS_TOL * max(size) * 2 + 1e-6
 
 
+@pytest.mark.parametrize("nc", [[3, 2, 2], [1, 2, 3]])
+def test_free_boundary_3d_far_outside_and_uneven_tessellations(cuda, nc):
+    """3D without zero boundary on tessellations of unequal sides, points up
+    to 4 domain lengths outside (the outside regions are unbounded): warped
+    points and both gradients against the FP64 oracle with the same
+    outside regions."""
+    from libcpab_amd import Cpab
+    T = Cpab(nc, backend="pytorch", device="gpu", zero_boundary=False)
+    p = T.get_params()
+    b = {"nc": np.array(nc), "planes": p.facet_planes, "nbr": p.facet_nbr, "amap": p.facet_amap,
+         "zero_boundary": False, "B": p.basis}
+    theta = _theta(T.get_theta_dim(), 2, 0.3, 41)
+    g = torch.Generator().manual_seed(42)
+    pts = torch.cat([torch.rand(3, 600, generator=g) * 1.4 - 0.2, torch.rand(3, 200, generator=g) * 9.0 - 4.0], 1)
+    th = theta.to(cuda).requires_grad_(True)
+    pp = pts.to(cuda).requires_grad_(True)
+    out = T.transform_grid(pp, th)
+    A = _A64(b["B"], theta, 3)
+    ref = _cf_forward(b, A, pts.double().numpy())
+    assert np.abs(out.detach().cpu().numpy() - ref).max() <= POS_TOL * max(1.0, np.abs(ref).max())
+    w = torch.randn(2, 3, pts.shape[1], generator=g)
+    (out * w.to(cuda)).sum().backward()
+    gA, gp = _cf_backward(b, A, pts.double().numpy(), w.double().numpy())
+    assert _rel(th.grad.cpu().double().numpy(), gA.reshape(2, -1) @ b["B"]) <= GRAD_RTOL
+    assert _rel(pp.grad.cpu().double().numpy(), gp.sum(0)) <= GRAD_RTOL
+
+
 @pytest.mark.parametrize("dsize,outsize", [([512, 512], [64, 64]), ([40, 50], [100, 90]), ([33, 7], [50, 5]),
                                            ([64, 64], [64, 64])])
 def test_transform_data_2d_tiles_and_lds_box(cuda, dsize, outsize):
