@@ -115,6 +115,23 @@ __device__ __forceinline__ void block_exclusive_scan(int* v, int n) {
     __syncthreads();
 }
 
+// q = n / d, rem = n - q d for 0 <= n < 2^20 and 0 < d < 2^20, rd = 1.0f / d:
+// the float product is within 1/8 of n / d, so truncating it is off by at
+// most one, which the remainder test corrects
+__device__ __forceinline__ int div_small(int n, int d, float rd, int& rem) {
+    int q = (int)((float)n * rd);
+    int r = n - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+    rem = r;
+    return q;
+}
+
 // output-order index t of round r of this thread in tile `tile`, -1 if outside
 template <int D>
 __device__ __forceinline__ int tile_t(const Launch& L, const Src& s, int tile, int r) {
@@ -858,10 +875,12 @@ __global__ __launch_bounds__(BLOCK) void k_transform_bwd(Launch L, Geo g, Src s,
 // dL/ddata of a 2D tile with saved state (C <= 4) goes through LDS when box
 // (box_cap words, nullable) holds the tile's footprint bins and point slots:
 // each point is binned by the data cell of its lower corner (one integer LDS
-// atomic), the bins are scanned into a point list, and every cell of the
-// footprint's bounding box gathers the weighted grad_out of the points in the
-// four bins that touch it and adds it to dL/ddata with one global atomic per
-// channel -- instead of four atomics per point and channel.
+// atomic in a counting pass before the integration), the bin counts are
+// scanned into offsets, every point writes its corner weights and grad_out
+// straight to its slot in bin order, and every cell of the footprint's
+// bounding box gathers the weighted grad_out of the points in the four bins
+// that touch it and adds it to dL/ddata with one global atomic per channel --
+// instead of four atomics per point and channel.
 template <int D, bool HAVES>
 __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, const Src& s, const Img& im,
                                                const CellTable<D>& tab, int tile, const float* __restrict__ data,
@@ -877,13 +896,13 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     bool use_box = false;
     int bx0 = 0, by0 = 0, bh = 1, cells = 0;
     const int nslot = L.ppt * BLOCK;
-    int* bins = reinterpret_cast<int*>(box);  // [cells + 1] counts, then offsets
-    int* code = bins;                         // [nslot] cell | rank << 11
-    int* list = bins;                         // [nslot] slots by bin
-    float2* fd2 = nullptr;                    // [nslot] corner weights (0 where a corner is clamped)
-    float4* gv4 = nullptr;                    // [nslot] grad_out of the channels
+    int* code = reinterpret_cast<int*>(box);  // [nslot] lower corner i0 << 15 | j0, then bin | rank << 11; -1: none
+    int* bins = code + nslot;                 // [cells + 1] counts, then offsets
+    float2* fd2 = nullptr;                    // [nslot] corner weights (0 where a corner is clamped), bin order
+    float4* gv4 = nullptr;                    // [nslot] grad_out of the channels, bin order
     if constexpr (D == 2 && HAVES) {
-        if (box && gdata) {  // uniform over the block
+        // uniform over the block; corner indices must fit the 15-bit fields of code
+        if (box && gdata && nslot <= box_cap && im.s0 <= 32768 && im.s1 <= 32768) {
             if (threadIdx.x == 0) {
                 s_bb[0] = INT_MAX;
                 s_bb[1] = INT_MIN;
@@ -894,11 +913,13 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             int lo0 = INT_MAX, hi0 = INT_MIN, lo1 = INT_MAX, hi1 = INT_MIN;
             for (int r = 0; r < L.ppt; ++r) {
                 const int t = tile_t<D>(L, s, tile, r);
+                code[r * BLOCK + threadIdx.x] = -1;
                 if (t < 0) continue;
                 const float y[2] = {sv[t], sv[(size_t)L.nP + t]};
                 int i0[3], i1[3];
                 float fd[3];
                 corners<2>(y, im, i0, i1, fd);
+                code[r * BLOCK + threadIdx.x] = i0[0] << 15 | i0[1];
                 lo0 = min(lo0, i0[0]);
                 hi0 = max(hi0, i1[0]);
                 lo1 = min(lo1, i0[1]);
@@ -919,23 +940,30 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             by0 = s_bb[2];
             bh = s_bb[3] - by0 + 1;
             cells = (s_bb[1] - bx0 + 1) * bh;
-            const long long slot0 = ((long long)cells + 1 + 2LL * nslot + 3) & ~3LL;  // 16-byte aligned
+            const long long slot0 = ((long long)nslot + cells + 1 + 3) & ~3LL;  // 16-byte aligned
             use_box = bx0 != INT_MAX && im.C <= 4 && cells < BIN_CELLS && slot0 + 6LL * nslot <= box_cap;
             __syncthreads();  // s_bb is reused by the next tile of a persistent block
             if (use_box) {
-                code = bins + cells + 1;
-                list = code + nslot;
                 fd2 = reinterpret_cast<float2*>(box + slot0);
                 gv4 = reinterpret_cast<float4*>(fd2 + nslot);
                 for (int i = threadIdx.x; i <= cells; i += BLOCK) bins[i] = 0;
                 __syncthreads();
+                // count the bins; each thread re-codes its own slots
+                for (int sl = threadIdx.x; sl < nslot; sl += BLOCK) {
+                    const int cd = code[sl];
+                    if (cd < 0) continue;
+                    const int k = ((cd >> 15) - bx0) * bh + ((cd & 0x7FFF) - by0);
+                    const int rank = atomicAdd(&bins[k], 1);
+                    code[sl] = k | (rank << 11);
+                }
+                __syncthreads();
+                block_exclusive_scan(bins, cells + 1);
             }
         }
     }
     for (int r = 0; r < L.ppt; ++r) {
         const int t = tile_t<D>(L, s, tile, r);
         const bool valid = t >= 0;
-        if (use_box && !valid) code[r * BLOCK + threadIdx.x] = -1;
         float q0[D], mu[D];
         int c0 = 0;
         ChunkLog lg;
@@ -994,12 +1022,11 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             }
             if constexpr (D == 2) {
                 if (use_box) {
-                    const int k = (i0[0] - bx0) * bh + (i0[1] - by0);
-                    const int rank = atomicAdd(&bins[k], 1);
-                    code[slot] = k | (rank << 11);
+                    const int cd = code[slot];
+                    const int pos = bins[cd & 2047] + (cd >> 11);
                     // a clamped corner pair (i1 == i0) puts its whole weight on i0
-                    fd2[slot] = make_float2(i1[0] != i0[0] ? fd[0] : 0.f, i1[1] != i0[1] ? fd[1] : 0.f);
-                    gv4[slot] = make_float4(gk[0], gk[1], gk[2], gk[3]);
+                    fd2[pos] = make_float2(i1[0] != i0[0] ? fd[0] : 0.f, i1[1] != i0[1] ? fd[1] : 0.f);
+                    gv4[pos] = make_float4(gk[0], gk[1], gk[2], gk[3]);
                 }
             }
             float w[D];
@@ -1017,20 +1044,16 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     }
     if (use_box) {
         __syncthreads();
-        block_exclusive_scan(bins, cells + 1);
-        for (int sl = threadIdx.x; sl < nslot; sl += BLOCK) {
-            const int cd = code[sl];
-            if (cd >= 0) list[bins[cd & 2047] + ((cd >> 11) & 0x3FFFF)] = sl;
-        }
-        __syncthreads();
         // Cell (i, j) is a corner of the points whose lower corner is in bins
         // (i - a, j - b), a, b in {0, 1}.  Bins (i - a, j - 1) and (i - a, j)
-        // are adjacent in the list, so the four bins are two ranges, walked
+        // are adjacent in bin order, so the four bins are two ranges, walked
         // as one loop (its trip count is the wave's largest total, not the
         // sum of four per-bin maxima): row a = 0 is [s0, e0) with bin
         // (i, j - 1) below m0, row a = 1 is [s1, e1) split at m1.
+        const float rbh = 1.0f / (float)bh;
         for (int k = threadIdx.x; k < cells; k += BLOCK) {
-            const int i = k / bh, j = k - i * bh;
+            int j;
+            const int i = div_small(k, bh, rbh, j);
             const int jb = j > 0 ? 1 : 0;
             const int s0 = bins[k - jb], m0 = bins[k], e0 = bins[k + 1];
             int s1 = 0, m1 = 0, e1 = 0;
@@ -1044,9 +1067,8 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
             for (int u = 0; u < n; ++u) {
                 const bool row0 = u < n0;
                 const int e = row0 ? s0 + u : s1 + (u - n0);
-                const int sl = list[e];
-                const float2 f = fd2[sl];
-                const float4 gq = gv4[sl];
+                const float2 f = fd2[e];
+                const float4 gq = gv4[e];
                 const float wx = row0 ? 1.f - f.x : f.x;
                 const float wy = e < (row0 ? m0 : m1) ? f.y : 1.f - f.y;
                 const float w = wx * wy;
@@ -1884,7 +1906,7 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
 }
 
 // LDS words for the dL/ddata bins and point slots of the fused 2D backward
-// (a 512-point tile needs 4096 + footprint cells + 1 words: up to ~2040
+// (a 512-point tile needs 3584 + footprint cells + 1 words: up to 2047
 // cells, a 32 x 16 tile at up to ~2x magnification)
 constexpr int BOX_FLOATS = 6144;
 

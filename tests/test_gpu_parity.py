@@ -198,11 +198,13 @@ def test_free_boundary_3d_far_outside_and_uneven_tessellations(cuda, nc):
 
 
 @pytest.mark.parametrize("dsize,outsize", [([512, 512], [64, 64]), ([40, 50], [100, 90]), ([33, 7], [50, 5]),
-                                           ([64, 64], [64, 64])])
+                                           ([64, 64], [64, 64]), ([96, 96], [64, 64])])
 def test_transform_data_2d_tiles_and_lds_box(cuda, dsize, outsize):
     """The 2D fused backward accumulates dL/ddata of each output tile in an LDS
-    box when its footprint fits (downsampling overflows it: global atomics;
-    a second axis below 8 runs untiled).  Gradients equal the unfused path's."""
+    box when its footprint fits, with the footprint's data staged beside it
+    when that fits too (96 -> 64: the bins fit, the data does not; 512 -> 64
+    overflows the bins: global atomics; a second axis below 8 runs untiled).
+    Gradients equal the unfused path's."""
     T, b = make_cpab("2d_4x4")
     theta = _theta(T.get_theta_dim(), 3, 0.3, 21)
     g = torch.Generator().manual_seed(22)
