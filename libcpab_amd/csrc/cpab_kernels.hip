@@ -1087,27 +1087,36 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
 
 // adjoint of k_transform_data.  gout [n_theta, C, outsize] in output order;
 // saved (nullable) from the forward.  im.gdata (nullable, zero-filled by the
-// caller) receives dL/ddata by atomic scatter.
+// caller) receives dL/ddata by atomic scatter.  Each block runs tpb
+// consecutive tiles of one theta into one row of partials (rows per theta:
+// ceil(tiles / tpb)), so the tables are staged and the partials written once
+// per tpb tiles.
 template <int D, bool HAVES>
 __global__ __launch_bounds__(BLOCK, (D == 3 && HAVES) ? 4 : 1) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
                                                               const float* __restrict__ gout,
                                                               const float* __restrict__ saved, float* __restrict__ part,
-                                                              int box_cap) {
+                                                              int box_cap, int tpb) {
     extern __shared__ float smem[];
     constexpr int P = D * (D + 1);
+    const int nbt = (L.tiles + tpb - 1) / tpb;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int th = lb / L.tiles, tile = lb % L.tiles;
+    const int th = lb / nbt, bt = lb % nbt;
     if (th >= L.n_theta) return;
     const int nC = g.nC;
     float* acc = smem + table_floats<D>(nC + g.nV);
     for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
-    fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
-                             im.gdata ? im.gdata + (size_t)th * im.C * dstride : nullptr,
-                             gout + (size_t)th * im.C * L.nP, HAVES ? saved + (size_t)th * saved_rows(D) * L.nP : nullptr,
-                             acc + (threadIdx.x >> 6) * nC * P, box_cap ? acc + WAVES * nC * P : nullptr, box_cap);
-    write_partials<D>(acc, part + ((size_t)th * L.tiles + tile) * nC * P, nC * P);
+    for (int k = 0; k < tpb; ++k) {
+        const int tile = bt * tpb + k;
+        if (tile >= L.tiles) break;
+        fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
+                                 im.gdata ? im.gdata + (size_t)th * im.C * dstride : nullptr,
+                                 gout + (size_t)th * im.C * L.nP,
+                                 HAVES ? saved + (size_t)th * saved_rows(D) * L.nP : nullptr,
+                                 acc + (threadIdx.x >> 6) * nC * P, box_cap ? acc + WAVES * nC * P : nullptr, box_cap);
+    }
+    write_partials<D>(acc, part + ((size_t)th * nbt + bt) * nC * P, nC * P);
 }
 
 // interpolation at an explicit grid [N, D, nP]; out [N, C, outsize], output order
@@ -1905,6 +1914,16 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
     return (ndim == 2 && tile_ty(2, g1) > 0 && ppt > 2) ? 2 : ppt;
 }
 
+// tiles per block of the fused 2D tiled backward: the most of 4, 2 that
+// still leaves 16 blocks per CU (256 CUs), else 1 (images2d backward: 0.81
+// ms at 1, 0.77 at 2 and 4, 0.78 at 8)
+int bwd_tiles_per_block(int ndim, int ty, int n_theta, int tiles) {
+    if (ndim != 2 || ty == 0) return 1;
+    for (int tpb = 4; tpb > 1; tpb >>= 1)
+        if ((long long)n_theta * ((tiles + tpb - 1) / tpb) >= 16LL * 256) return tpb;
+    return 1;
+}
+
 // LDS words for the dL/ddata bins and point slots of the fused 2D backward
 // (a 512-point tile needs 3584 + footprint cells + 1 words: up to 2047
 // cells, a 32 x 16 tile at up to ~2x magnification)
@@ -2377,18 +2396,20 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.tiles = tile_count(t->ndim, nP, s.g0, s.g1, L.ppt, s.g2);
     L.t_end = t_end;
     Geo g = make_geo(t);
-    dim3 grid(L.tiles * n_theta);
+    const int tpb = bwd_tiles_per_block(t->ndim, L.ty, n_theta, L.tiles);
+    const int nbt = (L.tiles + tpb - 1) / tpb;
+    dim3 grid(nbt * n_theta);
 #define LAUNCH_DB(DD)                                                                                                     \
     if (saved)                                                                                                             \
         hipLaunchKernelGGL((k_transform_data_bwd<DD, true>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, grad_out,    \
-                           saved, part, box_cap);                                                                          \
+                           saved, part, box_cap, tpb);                                                                     \
     else                                                                                                                   \
         hipLaunchKernelGGL((k_transform_data_bwd<DD, false>), grid, dim3(BLOCK), lds, stream, L, g, s, im, Aq, grad_out,   \
-                           saved, part, 0);
+                           saved, part, 0, tpb);
     CPAB_DISPATCH_D(t->ndim, LAUNCH_DB)
 #undef LAUNCH_DB
     if ((rc = status_from_hip())) return rc;
-    return launch_reduce(t, part, n_theta, L.tiles, grad_theta, nullptr, stream);
+    return launch_reduce(t, part, n_theta, nbt, grad_theta, nullptr, stream);
 }
 
 int cpab_findcell(int ndim, const int* nc, const float* points, int nP, int* idx, hipStream_t stream) {
