@@ -1166,7 +1166,7 @@ __device__ __forceinline__ float dpp_add(float v) {
     return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-// wave_total of P values at once, read from lane 63: every step is taken for
+// wave_total of P values at once, left in lane 63: every step is taken for
 // all P before the next, so the steps of one value do not wait on each
 // other.  The two row broadcasts run on all rows; the rows wave_total masks
 // out (0 and 2, then 0 and 1) end with other sums, but lane 63 -- which
@@ -1185,13 +1185,12 @@ __device__ __forceinline__ void wave_totals(float (&v)[P]) {
     for (int p = 0; p < P; ++p) v[p] = dpp_add<0x142>(v[p]);
 #pragma unroll
     for (int p = 0; p < P; ++p) v[p] = dpp_add<0x143>(v[p]);
-#pragma unroll
-    for (int p = 0; p < P; ++p) v[p] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[p]), 63));
 }
 
 // Adds g (P values) of every lane with cell >= 0 into acc[cell*P + p] of this
 // wave's LDS region; the lanes sharing a cell are summed across the wave in a
-// fixed order, one lane writes.  Must be reached by all lanes of the wave.
+// fixed order and lane 63 adds the sums (LDS float adds, issued in program
+// order by the one lane).  Must be reached by all lanes of the wave.
 template <int P>
 __device__ __forceinline__ void wave_accumulate(float* acc, int cell, const float (&g)[P]) {
     const int lane = __lane_id();
@@ -1205,9 +1204,9 @@ __device__ __forceinline__ void wave_accumulate(float* acc, int cell, const floa
 #pragma unroll
         for (int p = 0; p < P; ++p) tot[p] = mine ? g[p] : 0.f;
         wave_totals<P>(tot);
-        if (lane == leader) {
+        if (lane == 63) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) acc[c0 * P + p] += tot[p];
+            for (int p = 0; p < P; ++p) atomicAdd(&acc[c0 * P + p], tot[p]);
         }
     }
 }

@@ -224,6 +224,27 @@ def test_transform_data_2d_tiles_and_lds_box(cuda, dsize, outsize):
     assert _rel(th1.grad.cpu().double().numpy(), th2.grad.cpu().double().numpy()) <= 1e-4
 
 
+@pytest.mark.parametrize("n_theta", [64, 128])
+def test_transform_data_backward_with_several_tiles_per_block(cuda, n_theta):
+    """With enough thetas the fused 2D backward runs 2 (64 thetas) or 4 (128)
+    output tiles per block into one row of partials; both gradients equal
+    the unfused path's."""
+    T, b = make_cpab("2d_4x4")
+    theta = _theta(T.get_theta_dim(), n_theta, 0.3, 51)
+    g = torch.Generator().manual_seed(52)
+    data = torch.rand(n_theta, 3, 256, 256, generator=g).to(cuda)
+    gout = torch.randn(n_theta, 3, 256, 256, generator=g).to(cuda)
+    th1 = theta.to(cuda).requires_grad_(True)
+    d1 = data.clone().requires_grad_(True)
+    (T.transform_data(d1, th1, [256, 256]) * gout).sum().backward()
+    th2 = theta.to(cuda).requires_grad_(True)
+    d2 = data.clone().requires_grad_(True)
+    gt = T.transform_grid(T.uniform_meshgrid([256, 256]), th2)
+    (T.interpolate(d2, gt, [256, 256]) * gout).sum().backward()
+    assert torch.allclose(d1.grad, d2.grad, atol=1e-5, rtol=0)
+    assert _rel(th1.grad.cpu().double().numpy(), th2.grad.cpu().double().numpy()) <= 1e-4
+
+
 def test_identity_theta_is_exact_identity(cuda):
     T, b = make_cpab("2d_4x4")
     grid = T.uniform_meshgrid([64, 48])
@@ -234,6 +255,27 @@ def test_identity_theta_is_exact_identity(cuda):
     # x * (W-1) of a float32 linspace value is within an ulp of the pixel
     # index, so the identity reproduces the data to ~1e-6
     assert torch.allclose(T.transform_data(data, th, [64, 48]), data, atol=1e-5, rtol=0)
+
+
+def test_theta_gradient_is_bit_reproducible(cuda):
+    """dL/dtheta adds in a fixed order (wave sums in a fixed DPP order, one
+    lane per wave adding into that wave's own accumulator, tiles summed in
+    order): two runs on the same inputs give the same bits, for the fused
+    image path and the grid path."""
+    T, b = make_cpab("2d_4x4")
+    theta = _theta(T.get_theta_dim(), 6, 0.5, 31).to(cuda)
+    g = torch.Generator().manual_seed(32)
+    data = torch.rand(6, 3, 96, 80, generator=g).to(cuda)
+    grid = T.uniform_meshgrid([96, 80])
+    grads = []
+    for _ in range(2):
+        th = theta.clone().requires_grad_(True)
+        (T.transform_data(data, th, [96, 80]) ** 2).sum().backward()
+        th2 = theta.clone().requires_grad_(True)
+        (T.transform_grid(grid, th2) ** 2).sum().backward()
+        grads.append((th.grad.clone(), th2.grad.clone()))
+    assert torch.equal(grads[0][0], grads[1][0])
+    assert torch.equal(grads[0][1], grads[1][1])
 
 
 def _lookup_points(nd, nc, zb, rng):
