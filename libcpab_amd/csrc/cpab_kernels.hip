@@ -1970,9 +1970,9 @@ int launch_reduce(const cpab_tess_t* t, float* part, int n_theta, int tiles, flo
     const int max_slices = (Dt + 63) / 64;
     if (nslice > max_slices) nslice = max_slices;
     if (nslice < 1) nslice = 1;
-    if (nslice == 1) {
-        // enough thetas: one block per theta, as many waves (up to 16) as the
-        // LDS holds [nw + 1, Dt] floats for
+    if (nslice == 1 || tiles <= 64) {
+        // enough thetas, or few rows per theta: one block per theta, as many
+        // waves (up to 16) as the LDS holds [nw + 1, Dt] floats for
         int nw = RBLOCK / 64;
         while (nw > 1 && (size_t)(nw + 1) * Dt * sizeof(float) > LDS_LIMIT) nw >>= 1;
         if ((size_t)(nw + 1) * Dt * sizeof(float) <= LDS_LIMIT) {
