@@ -1085,12 +1085,23 @@ __device__ __forceinline__ void fused_bwd_tile(const Launch& L, const Geo& g, co
     }
 }
 
+// Blocks per theta of the fused backward with tpb tiles per block (tpb > 1
+// only on 2D tiled grids): a block takes tpb vertically adjacent tiles of
+// one tile column, so horizontally adjacent tiles -- which share the cache
+// lines of the output-order rows (16 columns = half a 128-byte line) -- run
+// in neighbouring blocks at the same time instead of one after the other in
+// one block (which fetched every such line twice).
+__host__ __device__ inline int bwd_blocks_per_theta(int tiles, int g1, int ty, int tpb) {
+    if (tpb == 1) return tiles;
+    const int tiles_y = (g1 + ty - 1) / ty, tiles_x = tiles / tiles_y;
+    return ((tiles_x + tpb - 1) / tpb) * tiles_y;
+}
+
 // adjoint of k_transform_data.  gout [n_theta, C, outsize] in output order;
 // saved (nullable) from the forward.  im.gdata (nullable, zero-filled by the
-// caller) receives dL/ddata by atomic scatter.  Each block runs tpb
-// consecutive tiles of one theta into one row of partials (rows per theta:
-// ceil(tiles / tpb)), so the tables are staged and the partials written once
-// per tpb tiles.
+// caller) receives dL/ddata by atomic scatter.  Each block runs tpb tiles of
+// one theta (bwd_blocks_per_theta) into one row of partials, so the tables
+// are staged and the partials written once per tpb tiles.
 template <int D, bool HAVES>
 __global__ __launch_bounds__(BLOCK, (D == 3 && HAVES) ? 4 : 1) void k_transform_data_bwd(Launch L, Geo g, Src s, Img im, const float* __restrict__ Aq,
                                                               const float* __restrict__ gout,
@@ -1098,18 +1109,21 @@ __global__ __launch_bounds__(BLOCK, (D == 3 && HAVES) ? 4 : 1) void k_transform_
                                                               int box_cap, int tpb) {
     extern __shared__ float smem[];
     constexpr int P = D * (D + 1);
-    const int nbt = (L.tiles + tpb - 1) / tpb;
+    const int nbt = bwd_blocks_per_theta(L.tiles, s.g1, L.ty, tpb);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int th = lb / nbt, bt = lb % nbt;
     if (th >= L.n_theta) return;
+    const int tiles_y = tpb > 1 ? (s.g1 + L.ty - 1) / L.ty : 1, tiles_x = L.tiles / tiles_y;
     const int nC = g.nC;
     float* acc = smem + table_floats<D>(nC + g.nV);
     for (int i = threadIdx.x; i < WAVES * nC * P; i += BLOCK) acc[i] = 0.f;
     const CellTable<D> tab = stage_tables<D>(smem, Aq + (size_t)th * nC * P, g);
     const size_t dstride = (size_t)im.s0 * (D > 1 ? im.s1 : 1) * (D > 2 ? im.s2 : 1);
     for (int k = 0; k < tpb; ++k) {
-        const int tile = bt * tpb + k;
-        if (tile >= L.tiles) break;
+        // tile row (bt / tiles_y) * tpb + k of tile column bt % tiles_y
+        const int trow = (bt / tiles_y) * tpb + k;
+        if (tpb > 1 && trow >= tiles_x) break;
+        const int tile = tpb > 1 ? trow * tiles_y + bt % tiles_y : bt;
         fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data + (size_t)th * im.C * dstride,
                                  im.gdata ? im.gdata + (size_t)th * im.C * dstride : nullptr,
                                  gout + (size_t)th * im.C * L.nP,
@@ -2397,7 +2411,7 @@ int cpab_transform_data_bwd(const cpab_tess_t* t, const float* theta, int n_thet
     L.t_end = t_end;
     Geo g = make_geo(t);
     const int tpb = bwd_tiles_per_block(t->ndim, L.ty, n_theta, L.tiles);
-    const int nbt = (L.tiles + tpb - 1) / tpb;
+    const int nbt = bwd_blocks_per_theta(L.tiles, s.g1, L.ty, tpb);
     dim3 grid(nbt * n_theta);
 #define LAUNCH_DB(DD)                                                                                                     \
     if (saved)                                                                                                             \
