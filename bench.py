@@ -111,7 +111,8 @@ def time_bwd_kernel(T, theta, data, target, size, reps):
         N.check(N.lib().cpab_transform_data_fwd(dev.ref, H._ptr(theta), n_theta, H._ptr(data), C, dsz, osz,
                                                 H._ptr(lin), 1.0, H._ptr(out), H._ptr(saved), H._ptr(ws), nb, stream))
 
-    bwd()
+    for _ in range(3):
+        bwd()
     torch.cuda.synchronize()
     ev[0].record()
     for _ in range(reps):
@@ -664,7 +665,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--kernel-reps", type=int, default=5)
+    ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--cpu-points", type=int, default=40960)
     ap.add_argument("--err-points", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
