@@ -224,23 +224,25 @@ def test_transform_data_2d_tiles_and_lds_box(cuda, dsize, outsize):
     assert _rel(th1.grad.cpu().double().numpy(), th2.grad.cpu().double().numpy()) <= 1e-4
 
 
-@pytest.mark.parametrize("n_theta", [64, 128])
-def test_transform_data_backward_with_several_tiles_per_block(cuda, n_theta):
-    """With enough thetas the fused 2D backward runs 2 (64 thetas) or 4 (128)
-    output tiles per block into one row of partials; both gradients equal
-    the unfused path's."""
+@pytest.mark.parametrize("n_theta,size", [(64, [256, 256]), (128, [256, 256]), (1400, [90, 40])])
+def test_transform_data_backward_with_several_tiles_per_block(cuda, n_theta, size):
+    """With enough thetas the fused 2D backward runs 2 (64 thetas of 256^2)
+    or 4 (128 of 256^2; 1400 of 90 x 40, whose 3 x 3 tiles leave a partial
+    group of tile rows and partial tiles) output tiles per block, down one
+    tile column, into one row of partials; both gradients equal the
+    unfused path's."""
     T, b = make_cpab("2d_4x4")
     theta = _theta(T.get_theta_dim(), n_theta, 0.3, 51)
     g = torch.Generator().manual_seed(52)
-    data = torch.rand(n_theta, 3, 256, 256, generator=g).to(cuda)
-    gout = torch.randn(n_theta, 3, 256, 256, generator=g).to(cuda)
+    data = torch.rand([n_theta, 3] + size, generator=g).to(cuda)
+    gout = torch.randn([n_theta, 3] + size, generator=g).to(cuda)
     th1 = theta.to(cuda).requires_grad_(True)
     d1 = data.clone().requires_grad_(True)
-    (T.transform_data(d1, th1, [256, 256]) * gout).sum().backward()
+    (T.transform_data(d1, th1, size) * gout).sum().backward()
     th2 = theta.to(cuda).requires_grad_(True)
     d2 = data.clone().requires_grad_(True)
-    gt = T.transform_grid(T.uniform_meshgrid([256, 256]), th2)
-    (T.interpolate(d2, gt, [256, 256]) * gout).sum().backward()
+    gt = T.transform_grid(T.uniform_meshgrid(size), th2)
+    (T.interpolate(d2, gt, size) * gout).sum().backward()
     assert torch.allclose(d1.grad, d2.grad, atol=1e-5, rtol=0)
     assert _rel(th1.grad.cpu().double().numpy(), th2.grad.cpu().double().numpy()) <= 1e-4
 
