@@ -228,14 +228,17 @@ size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, const int* outsize, int 
 
 /* Splits every instance into tiles of 256*ppt points (compact 2D tiles for 2D
  * grids whose second axis has at least 8 points): writes inst[i].item0 /
- * .tiles and item_inst[item] (host array of max_items ints).  Returns the
- * number of work items, or -CPAB_ERR_* .  With item_inst == NULL it only
- * counts the items. */
+ * .tiles and item_inst[item] (host array of max_items ints).  The items of
+ * one instance are consecutive; instances are grouped by the LDS their
+ * backward needs (three classes, in class order).  Returns the number of
+ * work items, or -CPAB_ERR_* .  With item_inst == NULL it only counts the
+ * items. */
 int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, int max_items);
 
 /* One persistent launch over all work items (blocks = grid size; workers pull
  * items from an atomic queue at counter[0], reset inside the call by a
- * kernel, so the sequence can be captured into a graph), preceded by the
+ * kernel, so the sequence can be captured into a graph; counter holds at
+ * least 33 ints), preceded by the
  * per-instance A = B theta kernel.  No reference counterpart (the reference
  * handles one tessellation and one size per call,
  * libcpab/pytorch/transformer_cuda.cu:18-119). */
@@ -244,7 +247,9 @@ int cpab_batch_fwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
                    hipStream_t stream);
 
 /* Backward of cpab_batch_fwd: dL/ddata (if grad_data), dL/dtheta of every
- * instance (per-item partials reduced per instance in a fixed order). */
+ * instance (per-item partials reduced per instance in a fixed order).  One
+ * persistent launch per LDS class of the plan, each with the LDS of its
+ * largest tessellation and its own queue (counter[0], [16], [32]). */
 int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_inst, int n_inst,
                    const int* dev_item_inst, int n_items, int ppt, int blocks, float t_end, int* counter,
                    hipStream_t stream);

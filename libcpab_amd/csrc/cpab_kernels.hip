@@ -1420,6 +1420,10 @@ __device__ __forceinline__ void item_args(const cpab_instance_t* I, int ppt, flo
 // mostly belong to one instance, whose cell tables are then staged once for
 // all of them.
 constexpr int GRAB = 4;
+// The backward's LDS (tables + per-wave accumulators + dL/ddata box) is sized
+// for the largest tessellation of its launch; instances are grouped into
+// classes that fit 4, 3 and 2 blocks per CU, one launch per class.
+constexpr int BATCH_CLASSES = 3;
 
 __device__ __forceinline__ int next_grab(int* counter, int* s_item) {
     if (threadIdx.x == 0) *s_item = atomicAdd(counter, GRAB);
@@ -1429,8 +1433,10 @@ __device__ __forceinline__ int next_grab(int* counter, int* s_item) {
     return item;
 }
 
+// the work queues at counter[0], counter[16] and counter[32] (one per LDS
+// class of the backward, a cache line apart)
 __global__ void k_counter_reset(int* counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < BATCH_CLASSES) counter[threadIdx.x * 16] = 0;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fill_zero(float* p, size_t n) {
@@ -1522,17 +1528,19 @@ __device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, flo
 }
 
 template <bool HAVES, int MAXD>
+// Work items [item_lo, item_hi) (one LDS class of instances, see
+// cpab_batch_plan), pulled from their own queue at counter.
 __global__ __launch_bounds__(BLOCK, MAXD == 2 ? 4 : 1) void k_batch_bwd(const cpab_instance_t* __restrict__ inst,
-                                                     const int* __restrict__ item_inst, int n_items, int ppt,
-                                                     float t_end, int* counter, int acc_offset, int box_offset,
-                                                     int box_cap) {
+                                                     const int* __restrict__ item_inst, int item_lo, int item_hi,
+                                                     int ppt, float t_end, int* counter, int acc_offset,
+                                                     int box_offset, int box_cap) {
     extern __shared__ float smem[];
     __shared__ int s_item;
     int cur = -1;
     for (;;) {
-        const int first = next_grab(counter, &s_item);
-        if (first >= n_items) break;
-        const int last = first + GRAB < n_items ? first + GRAB : n_items;
+        const int first = item_lo + next_grab(counter, &s_item);
+        if (first >= item_hi) break;
+        const int last = first + GRAB < item_hi ? first + GRAB : item_hi;
         for (int item = first; item < last; ++item) {
             // the previous item's waves are done with the tables and accumulators
             if (item != first) __syncthreads();
@@ -2525,6 +2533,16 @@ size_t cpab_batch_workspace_bytes(const cpab_tess_t* t, const int* outsize, int 
     return batched > plain ? batched : plain;
 }
 
+// LDS class of an instance in the batched backward: its tables and
+// per-wave accumulators next to a full dL/ddata box fit 4 (class 0), 3 (1)
+// or 2 or fewer (2) blocks per CU
+static int batch_class(const cpab_instance_t& I) {
+    const int nd = I.tess.ndim;
+    const size_t need = align256(lds_tables(nd, I.tess.nC + I.tess.nV)) +
+                        (size_t)WAVES * I.tess.nC * nd * (nd + 1) * 4 + (size_t)BOX_FLOATS * 4 + 256;
+    return need <= LDS_LIMIT / 4 ? 0 : (need <= LDS_LIMIT / 3 ? 1 : 2);
+}
+
 int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, int max_items) {
     if (!inst || n_inst < 0 || ppt < 1) return -CPAB_ERR_ARG;
     long long item = 0;
@@ -2538,16 +2556,20 @@ int cpab_batch_plan(cpab_instance_t* inst, int n_inst, int ppt, int* item_inst, 
     }
     if (!item_inst) return (int)item;  // count only
     if (item > max_items) return -CPAB_ERR_WORKSPACE;
+    // the items of each backward LDS class together, classes in order
     item = 0;
-    for (int i = 0; i < n_inst; ++i) {
-        cpab_instance_t& I = inst[i];
-        const int nd = I.tess.ndim;
-        const int tiles = tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt, nd > 2 ? I.outsize[2] : 1);
-        I.item0 = (int)item;
-        I.tiles = tiles;
-        for (int k = 0; k < tiles; ++k) item_inst[item + k] = i;
-        item += tiles;
-    }
+    for (int cls = 0; cls < BATCH_CLASSES; ++cls)
+        for (int i = 0; i < n_inst; ++i) {
+            cpab_instance_t& I = inst[i];
+            if (batch_class(I) != cls) continue;
+            const int nd = I.tess.ndim;
+            const int tiles =
+                tile_count(nd, I.nP, I.outsize[0], nd > 1 ? I.outsize[1] : 1, ppt, nd > 2 ? I.outsize[2] : 1);
+            I.item0 = (int)item;
+            I.tiles = tiles;
+            for (int k = 0; k < tiles; ++k) item_inst[item + k] = i;
+            item += tiles;
+        }
     return (int)item;
 }
 
@@ -2621,25 +2643,58 @@ int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
     hipLaunchKernelGGL(k_batch_A, dim3((maxDt + BLOCK - 1) / BLOCK, n_inst), dim3(BLOCK), maxd * sizeof(float), stream,
                        dev_inst);
     hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, stream, counter);
-    const int acc_off = (int)(lds_tab / 4);
-    // LDS box for dL/ddata when some 2D instance has a tiled grid and wants it
-    bool want_box = false;
+    // the item range of each LDS class (contiguous, as cpab_batch_plan lays
+    // them out); a plan in another order runs as one class
+    int lo[BATCH_CLASSES], hi[BATCH_CLASSES];
+    long long cnt[BATCH_CLASSES] = {0, 0, 0};
+    for (int c = 0; c < BATCH_CLASSES; ++c) {
+        lo[c] = INT_MAX;
+        hi[c] = 0;
+    }
     for (int i = 0; i < n_inst; ++i) {
         const cpab_instance_t& I = host_inst[i];
-        want_box = want_box || (I.tess.ndim == 2 && I.grad_data && tile_ty(2, I.outsize[1]) > 0);
+        const int c = batch_class(I);
+        lo[c] = I.item0 < lo[c] ? I.item0 : lo[c];
+        hi[c] = I.item0 + I.tiles > hi[c] ? I.item0 + I.tiles : hi[c];
+        cnt[c] += I.tiles;
     }
-    const int box_cap = (have && want_box && lds_tab + lds_acc + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT) ? BOX_FLOATS : 0;
-    const int box_off = (int)((lds_tab + lds_acc) / 4);
-    const size_t lds = lds_tab + lds_acc + (size_t)box_cap * 4;
-    int maxnd = 1;
-    for (int i = 0; i < n_inst; ++i) maxnd = host_inst[i].tess.ndim > maxnd ? host_inst[i].tess.ndim : maxnd;
-#define LAUNCH_BB(S, M, CAP)                                                                                     \
-    hipLaunchKernelGGL((k_batch_bwd<S, M>), dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst,    \
-                       n_items, ppt, t_end, counter, acc_off, box_off, CAP)
-    if (have) {
-        if (maxnd == 3) LAUNCH_BB(true, 3, box_cap); else LAUNCH_BB(true, 2, box_cap);
-    } else {
-        if (maxnd == 3) LAUNCH_BB(false, 3, 0); else LAUNCH_BB(false, 2, 0);
+    bool split = true;
+    for (int c = 0; c < BATCH_CLASSES; ++c)
+        split = split && (cnt[c] == 0 || (hi[c] - lo[c] == cnt[c] && hi[c] <= n_items));
+    for (int c = 0; c < BATCH_CLASSES; ++c) {
+        if (split && cnt[c] == 0) continue;
+        if (!split && c > 0) break;
+        const int item_lo = split ? lo[c] : 0, item_hi = split ? hi[c] : n_items;
+        // LDS of this launch: the largest tables and accumulators of its instances
+        size_t ltab = 0, lacc = 0;
+        bool want_box = false;
+        int maxnd = 1;
+        for (int i = 0; i < n_inst; ++i) {
+            const cpab_instance_t& I = host_inst[i];
+            if (split && batch_class(I) != c) continue;
+            const int nd = I.tess.ndim;
+            maxnd = nd > maxnd ? nd : maxnd;
+            const size_t tb = lds_tables(nd, I.tess.nC + I.tess.nV);
+            const size_t ab = (size_t)WAVES * I.tess.nC * nd * (nd + 1) * 4;
+            ltab = tb > ltab ? tb : ltab;
+            lacc = ab > lacc ? ab : lacc;
+            want_box = want_box || (nd == 2 && I.grad_data && tile_ty(2, I.outsize[1]) > 0);
+        }
+        ltab = align256(ltab);
+        const int acc_off = (int)(ltab / 4);
+        // LDS box for dL/ddata when some 2D instance has a tiled grid and wants it
+        const int box_cap = (have && want_box && ltab + lacc + (size_t)BOX_FLOATS * 4 <= LDS_LIMIT) ? BOX_FLOATS : 0;
+        const int box_off = (int)((ltab + lacc) / 4);
+        const size_t lds = ltab + lacc + (size_t)box_cap * 4;
+        int* q = counter + 16 * c;
+#define LAUNCH_BB(S, M, CAP)                                                                                      \
+    hipLaunchKernelGGL((k_batch_bwd<S, M>), dim3(blocks), dim3(BLOCK), lds, stream, dev_inst, dev_item_inst,     \
+                       item_lo, item_hi, ppt, t_end, q, acc_off, box_off, CAP)
+        if (have) {
+            if (maxnd == 3) LAUNCH_BB(true, 3, box_cap); else LAUNCH_BB(true, 2, box_cap);
+        } else {
+            if (maxnd == 3) LAUNCH_BB(false, 3, 0); else LAUNCH_BB(false, 2, 0);
+        }
     }
 #undef LAUNCH_BB
     if ((rc = status_from_hip())) return rc;
