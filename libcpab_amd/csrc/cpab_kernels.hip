@@ -1213,9 +1213,15 @@ constexpr int RBLOCK = 1024;
 // order, scaled to x-space (gA, nullable, receives dL/dA), and B^T dL/dA is
 // split over the waves by rows of B and added in wave order.  sg holds
 // [nw + 1, Dt] floats.
+// Row j of the partials that reduce_theta sums (j < its row count).
+struct AllRows {
+    __device__ __forceinline__ int operator()(int j) const { return j; }
+};
+
+template <class Rows = AllRows>
 __device__ __forceinline__ void reduce_theta(float* sg, const float* __restrict__ pp, int tiles, int Dt, int ndim,
                                              const int (&nc)[3], const float* __restrict__ B, int d,
-                                             float* __restrict__ gth, float* __restrict__ gA) {
+                                             float* __restrict__ gth, float* __restrict__ gA, Rows rows = Rows()) {
     const int nw = blockDim.x >> 6;
     const int P = ndim * (ndim + 1);
     // wave w sums tiles w, w + nw, ... (two chains for overlap)
@@ -1224,10 +1230,10 @@ __device__ __forceinline__ void reduce_theta(float* sg, const float* __restrict_
         float s0 = 0.f, s1 = 0.f;
         int t = w;
         for (; t + nw < tiles; t += 2 * nw) {
-            s0 += pp[(size_t)t * Dt + i];
-            s1 += pp[(size_t)(t + nw) * Dt + i];
+            s0 += pp[(size_t)rows(t) * Dt + i];
+            s1 += pp[(size_t)rows(t + nw) * Dt + i];
         }
-        if (t < tiles) s0 += pp[(size_t)t * Dt + i];
+        if (t < tiles) s0 += pp[(size_t)rows(t) * Dt + i];
         sg[w * Dt + i] = s0 + s1;
     }
     __syncthreads();
@@ -1511,20 +1517,28 @@ __global__ __launch_bounds__(BLOCK) void k_batch_fwd(const cpab_instance_t* __re
     }
 }
 
+// One work item of the batched backward.  A run of consecutive items of one
+// instance inside one grab accumulates into the same LDS accumulators
+// (cleared at the run's first item) and writes one row of partials, at the
+// run's first tile, after its last item.
 template <int D, bool HAVES>
 __device__ __forceinline__ void batch_bwd_item(float* smem, float* acc_base, float* box, int box_cap,
-                                               const cpab_instance_t* I, bool restage, const Launch& L, const Geo& g,
-                                               const Src& s, const Img& im, int tile) {
+                                               const cpab_instance_t* I, bool restage, bool run_start, bool run_end,
+                                               int row, const Launch& L, const Geo& g, const Src& s, const Img& im,
+                                               int tile) {
     constexpr int P = D * (D + 1);
     const int nCP = g.nC * P;
     float* acc = acc_base;
-    for (int i = threadIdx.x; i < WAVES * nCP; i += BLOCK) acc[i] = 0.f;
+    if (run_start)
+        for (int i = threadIdx.x; i < WAVES * nCP; i += BLOCK) acc[i] = 0.f;
     const CellTable<D> tab = restage ? stage_tables<D>(smem, static_cast<const float*>(I->ws), g) : table_view<D>(smem, g.nC + g.nV);
     if (!restage) __syncthreads();
     fused_bwd_tile<D, HAVES>(L, g, s, im, tab, tile, im.data, im.gdata, I->grad_out, I->saved,
                              acc + (threadIdx.x >> 6) * nCP, box, box_cap);
-    float* part = reinterpret_cast<float*>(static_cast<char*>(I->ws) + align256_hd((size_t)nCP * 4));
-    write_partials<D>(acc, part + (size_t)tile * nCP, nCP);
+    if (run_end) {
+        float* part = reinterpret_cast<float*>(static_cast<char*>(I->ws) + align256_hd((size_t)nCP * 4));
+        write_partials<D>(acc, part + (size_t)row * nCP, nCP);
+    }
 }
 
 template <bool HAVES, int MAXD>
@@ -1537,10 +1551,16 @@ __global__ __launch_bounds__(BLOCK, MAXD == 2 ? 4 : 1) void k_batch_bwd(const cp
     extern __shared__ float smem[];
     __shared__ int s_item;
     int cur = -1;
+    // grabs are the GRAB-aligned item groups [k GRAB, (k + 1) GRAB) that meet
+    // [item_lo, item_hi), so the rows of partials an instance gets written
+    // (batch_rows) do not depend on the class ranges
+    const int base = item_lo & ~(GRAB - 1);
     for (;;) {
-        const int first = item_lo + next_grab(counter, &s_item);
-        if (first >= item_hi) break;
-        const int last = first + GRAB < item_hi ? first + GRAB : item_hi;
+        const int g0 = base + next_grab(counter, &s_item);
+        if (g0 >= item_hi) break;
+        const int first = g0 > item_lo ? g0 : item_lo;
+        const int last = g0 + GRAB < item_hi ? g0 + GRAB : item_hi;
+        int row = 0;
         for (int item = first; item < last; ++item) {
             // the previous item's waves are done with the tables and accumulators
             if (item != first) __syncthreads();
@@ -1554,18 +1574,42 @@ __global__ __launch_bounds__(BLOCK, MAXD == 2 ? 4 : 1) void k_batch_bwd(const cp
             const int tile = item - I->item0;
             const bool restage = ii != cur;
             cur = ii;
+            const bool run_start = item == first || tile == 0;
+            const bool run_end = item == last - 1 || tile == I->tiles - 1;
+            if (run_start) row = tile;
             float* acc = smem + acc_offset;  // after the largest table of the batch
             float* box = box_cap ? smem + box_offset : nullptr;
             switch (g.ndim) {
-                case 1: batch_bwd_item<1, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
-                case 2: batch_bwd_item<2, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile); break;
+                case 1:
+                    batch_bwd_item<1, HAVES>(smem, acc, box, box_cap, I, restage, run_start, run_end, row, L, g, s, im,
+                                             tile);
+                    break;
+                case 2:
+                    batch_bwd_item<2, HAVES>(smem, acc, box, box_cap, I, restage, run_start, run_end, row, L, g, s, im,
+                                             tile);
+                    break;
                 default:
                     if constexpr (MAXD == 3)
-                        batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, L, g, s, im, tile);
+                        batch_bwd_item<3, HAVES>(smem, acc, box, box_cap, I, restage, run_start, run_end, row, L, g, s,
+                                                 im, tile);
                     break;
             }
         }
     }
+}
+
+// The rows of partials the batched backward writes for an instance whose
+// items start at item0: the first item's, then one per GRAB-aligned item
+// after it -- j = 0, 1, ... maps to tile rows 0, a, a + GRAB, ... with a the
+// distance to the next multiple of GRAB (GRAB when item0 is one).
+struct RunRows {
+    int a;
+    __device__ __forceinline__ int operator()(int j) const { return j == 0 ? 0 : a + GRAB * (j - 1); }
+};
+__device__ __forceinline__ int batch_rows(int item0, int tiles, RunRows& rr) {
+    const int r = item0 & (GRAB - 1);
+    rr.a = r ? GRAB - r : GRAB;
+    return 1 + (rr.a < tiles ? (tiles - 1 - rr.a) / GRAB + 1 : 0);
 }
 
 // per instance: dL/dtheta = B^T (sum over its items of the partials, in x-space)
@@ -1576,7 +1620,9 @@ __global__ __launch_bounds__(RBLOCK) void k_batch_reduce(const cpab_instance_t* 
     const int Dt = I->tess.nC * nd * (nd + 1);
     const float* part = reinterpret_cast<const float*>(static_cast<const char*>(I->ws) + align256_hd((size_t)Dt * 4));
     const int nc[3] = {I->tess.nc[0], I->tess.nc[1], I->tess.nc[2]};
-    reduce_theta(sg, part, I->tiles, Dt, nd, nc, I->tess.B, I->tess.d, I->grad_theta, nullptr);
+    RunRows rr;
+    const int nrows = batch_rows(I->item0, I->tiles, rr);
+    reduce_theta(sg, part, nrows, Dt, nd, nc, I->tess.B, I->tess.d, I->grad_theta, nullptr, rr);
 }
 
 // ---------------------------------------------------------------------------
