@@ -249,7 +249,10 @@ int cpab_batch_fwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
 /* Backward of cpab_batch_fwd: dL/ddata (if grad_data), dL/dtheta of every
  * instance (per-item partials reduced per instance in a fixed order).  One
  * persistent launch per LDS class of the plan, each with the LDS of its
- * largest tessellation and its own queue (counter[0], [16], [32]). */
+ * largest tessellation and its own queue (counter[0], [16], [32]).  When
+ * every instance has saved state, it is the state of the preceding
+ * cpab_batch_fwd on the same instances and thetas, and the backward takes
+ * A = B theta from the workspaces where that forward left it. */
 int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_inst, int n_inst,
                    const int* dev_item_inst, int n_items, int ppt, int blocks, float t_end, int* counter,
                    hipStream_t stream);

@@ -2686,8 +2686,11 @@ int cpab_batch_bwd(const cpab_instance_t* host_inst, const cpab_instance_t* dev_
     if (rc) return rc;
     bool have = true;
     for (int i = 0; i < n_inst; ++i) have = have && host_inst[i].saved != nullptr;
-    hipLaunchKernelGGL(k_batch_A, dim3((maxDt + BLOCK - 1) / BLOCK, n_inst), dim3(BLOCK), maxd * sizeof(float), stream,
-                       dev_inst);
+    // with the forward's saved state, its A_q is still in every instance's
+    // workspace (the backward only writes the partials after it)
+    if (!have)
+        hipLaunchKernelGGL(k_batch_A, dim3((maxDt + BLOCK - 1) / BLOCK, n_inst), dim3(BLOCK), maxd * sizeof(float),
+                           stream, dev_inst);
     hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, stream, counter);
     // the item range of each LDS class (contiguous, as cpab_batch_plan lays
     // them out); a plan in another order runs as one class
