@@ -1991,8 +1991,8 @@ int pick_ppt_bwd(int ndim, int n_theta, int nP, int g1) {
 }
 
 // tiles per block of the fused 2D tiled backward: the most of 4, 2 that
-// still leaves 16 blocks per CU (256 CUs), else 1 (images2d backward: 0.81
-// ms at 1, 0.77 at 2 and 4, 0.78 at 8)
+// still leaves 16 blocks per CU (256 CUs), else 1 (images2d backward, tiles
+// down a column: 0.81 ms at 1, 0.766 at 2, 0.741 at 4, 0.745 at 8)
 int bwd_tiles_per_block(int ndim, int ty, int n_theta, int tiles) {
     if (ndim != 2 || ty == 0) return 1;
     for (int tpb = 4; tpb > 1; tpb >>= 1)
