@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CPAB_ABI_VERSION 6
+#define CPAB_ABI_VERSION 7
 
 #define CPAB_OK 0
 #define CPAB_ERR_ARG 1

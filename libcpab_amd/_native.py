@@ -81,7 +81,7 @@ _SIGNATURES = {
 EXPORTED = tuple(_SIGNATURES)
 
 # CPAB_ABI_VERSION of include/cpab_hip.h that these signatures and structs mirror
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 # static scratch per GPU of cpab_transform_data_fwd_noalloc (CPAB_NOALLOC_SCRATCH_BYTES)
 CPAB_NOALLOC_SCRATCH_BYTES = 4 << 20

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = N.lib()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.cpab_abi_version() == N.ABI_VERSION == 6
+    assert lib.cpab_abi_version() == N.ABI_VERSION == 7
     assert lib.cpab_status_string(0) == b"ok"
     assert b"workspace" in lib.cpab_status_string(3)
 
