@@ -146,6 +146,43 @@ def test_cpab_facade_contract():
     assert S.get_theta_dim() == [34, 62]
 
 
+def test_batch_plan_groups_instances_by_backward_lds_class():
+    """cpab_batch_plan (host code) lays out each instance's work items
+    consecutively and the instances class by class: those whose backward
+    tables, accumulators and dL/ddata box fit 4 blocks per CU ([2,2] and a
+    1D tessellation), then 3 ([6,6]: 144 simplices), then 2 ([8,8]: 256),
+    each class in instance order.  The plan reads no device memory, so host
+    stand-ins serve for the table pointers."""
+    lib = N.lib()
+    dummy = (ctypes.c_int * 4)()
+    specs = [([8, 8], [100, 90]), ([2, 2], [64, 48]), ([6, 6], [40, 40]), ([10], [700]), ([8, 8], [30, 20]),
+             ([2, 2], [33, 7])]
+    insts = (N.CpabInstance * len(specs))()
+    for I, (nc, osz) in zip(insts, specs):
+        nd = len(nc)
+        I.tess.ndim = nd
+        I.tess.nc = N.int3(nc)
+        I.tess.nC = int(np.prod(nc)) * (4 if nd == 2 else 1)
+        I.tess.d = 1
+        I.tess.planes = ctypes.addressof(dummy)
+        I.tess.nbr = ctypes.addressof(dummy)
+        I.outsize = N.int3(osz)
+        I.nP = int(np.prod(osz))
+    ptr = ctypes.cast(insts, ctypes.c_void_p)
+    total = lib.cpab_batch_plan(ptr, len(specs), 2, None, 0)
+    assert total > 0
+    items = (ctypes.c_int * total)()
+    assert lib.cpab_batch_plan(ptr, len(specs), 2, ctypes.cast(items, ctypes.c_void_p), total) == total
+    items = np.frombuffer(items, dtype=np.int32)
+    # every instance's items are one run at item0
+    for i, I in enumerate(insts):
+        assert I.tiles > 0 and (items[I.item0:I.item0 + I.tiles] == i).all()
+    assert sum(I.tiles for I in insts) == total
+    # runs in class order, instance order inside a class
+    order = [int(items[k]) for k in range(total) if k == 0 or items[k] != items[k - 1]]
+    assert order == [1, 3, 5, 2, 0, 4]
+
+
 def test_meshgrid_order_matches_reference():
     from libcpab_amd.pytorch.functions import uniform_meshgrid
     import oracle as O
